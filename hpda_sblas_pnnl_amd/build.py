@@ -1,0 +1,77 @@
+"""Builds the engine in-tree for gfx950.
+
+* ``libsblas.so``     -- HIP kernels + C ABI (include/sblas.h), built with hipcc.
+* ``libsblas_gen.so`` -- seeded synthetic-matrix generators (host C++, OpenMP).
+
+Run ``python -m hpda_sblas_pnnl_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libsblas.so")
+GEN_LIB = os.path.join(PKG, "libsblas_gen.so")
+ARCH = os.environ.get("SBLAS_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = [
+    "sblas_vector.hip",
+    "sblas_tiles.hip",
+    "plan.cpp",
+    "dropin.cpp",
+    "partition.cpp",
+]
+GEN_SOURCES = ["gen.cpp"]
+
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed FP32 instructions stay off (device code)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-I", INCLUDE, "-I", CSRC,
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    path = os.path.join(CSRC, src)
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    _run([HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
+    return obj
+
+
+def build(verbose: bool = False) -> None:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with cf.ThreadPoolExecutor(max_workers=min(len(srcs), 6)) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    gtmp = GEN_LIB + ".tmp"
+    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-fopenmp", "-shared",
+          "-Wall", "-o", gtmp, *[os.path.join(CSRC, s) for s in GEN_SOURCES]])
+    os.replace(gtmp, GEN_LIB)
+    if verbose:
+        print(f"built {LIB}\nbuilt {GEN_LIB}")
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

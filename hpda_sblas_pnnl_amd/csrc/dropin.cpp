@@ -1,0 +1,485 @@
+// dropin.cpp -- the reference's C entry points (include/spmv_kernel.h:211-296)
+// on one MI355X.
+//
+// Each call is synchronous and stateless like the reference's: the caller's
+// host arrays are uploaded, a plan is built with the entry point's partition
+// rule at the configured chunk count (ngpu must be 1; chunks come from
+// SBLAS_CHUNKS, default 1), the kernel runs on a private stream, split rows
+// are merged on the device and y is copied back.  spmv_ret holds the times of
+// those steps in seconds: part_time (partition + plan), comm_time (uploads and
+// download), comp_time (kernels), merg_time (device merge of split rows).
+//
+// Semantics follow y = alpha*A*x + beta*y for every row, including the rows
+// the reference leaves unwritten or scales twice (empty rows at chunk edges,
+// CSC/COO merges that add beta*y per chunk); beta == 0 does not read y.
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace sblas {
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+spmv_ret nan_ret() {
+  const double q = std::numeric_limits<double>::quiet_NaN();
+  return spmv_ret{q, q, q, q, q};
+}
+
+int env_int(const char *name, int fallback) {
+  const char *e = std::getenv(name);
+  return (e && *e) ? std::atoi(e) : fallback;
+}
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = false;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) return;
+    ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+struct DeviceArrays {
+  std::vector<void *> ptrs;
+  ~DeviceArrays() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T *alloc(size_t count) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * (count ? count : 1)) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return static_cast<T *>(p);
+  }
+};
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  double seconds() {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms / 1000.0;
+  }
+};
+
+// CSR core of every entry point: host arrays in, y out.
+spmv_ret run_csr(int m, int n, int nnz, const double *alpha, const double *val, const int *rp,
+                 const int *col, const double *x, const double *beta, double *y, int kernel,
+                 int part_mode, int nb, int q, double host_prep_s) {
+  spmv_ret ret{0, 0, 0, 0, 0};
+  const double t_part0 = now_s();
+  DeviceGuard guard(env_int("SBLAS_DEVICE", 0));
+  if (!guard.ok) {
+    set_error(SBLAS_ERR_NO_DEVICE, "no HIP device");
+    return nan_ret();
+  }
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "stream creation failed");
+    return nan_ret();
+  }
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } sg{s};
+  DeviceArrays mem;
+  int *d_rp = mem.alloc<int>(static_cast<size_t>(m) + 1);
+  int *d_col = mem.alloc<int>(nnz);
+  double *d_val = mem.alloc<double>(nnz);
+  double *d_x = mem.alloc<double>(n);
+  double *d_y = mem.alloc<double>(m);
+  if (!d_rp || !d_col || !d_val || !d_x || !d_y) {
+    set_error(SBLAS_ERR_ALLOC, "device allocation failed");
+    return nan_ret();
+  }
+  EventPair up, comp, merge, down;
+  (void)hipEventRecord(up.a, s);
+  bool ok = hipMemcpyAsync(d_rp, rp, sizeof(int) * (static_cast<size_t>(m) + 1),
+                           hipMemcpyHostToDevice, s) == hipSuccess;
+  if (nnz > 0) {
+    ok = ok && hipMemcpyAsync(d_col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(d_val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s) == hipSuccess;
+  }
+  if (n > 0) ok = ok && hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, s) == hipSuccess;
+  if (m > 0) ok = ok && hipMemcpyAsync(d_y, y, sizeof(double) * m, hipMemcpyHostToDevice, s) == hipSuccess;
+  (void)hipEventRecord(up.b, s);
+  if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "upload failed");
+    return nan_ret();
+  }
+  sblas_opts o;
+  sblas_opts_default(&o);
+  o.kernel = kernel;
+  o.part_mode = part_mode;
+  o.nb = nb;
+  o.streams = q;
+  o.chunks = std::max(1, env_int("SBLAS_CHUNKS", 1));
+  if (part_mode == SBLAS_PART_ROWBLOCK) o.chunks = std::max(1, env_int("SBLAS_CHUNKS", 1));
+  sblas_plan plan = nullptr;
+  const double t_plan0 = now_s();
+  int st = sblas_plan_create_csr(m, n, nnz, d_rp, d_col, d_val, SBLAS_VAL_F64, &o, &plan);
+  if (st != SBLAS_OK) return nan_ret();
+  struct PlanGuard {
+    sblas_plan p;
+    ~PlanGuard() { sblas_plan_destroy(p); }
+  } pg{plan};
+  ret.part_time = host_prep_s + (now_s() - t_plan0) + (t_plan0 - t_part0) * 0.0;
+  // kernels, then the split-row merge, each between its own events
+  Plan &p = plan->impl;
+  const int groups = p.ngroups;
+  p.ngroups = 0;
+  (void)hipEventRecord(comp.a, s);
+  st = p.run_spmv(*alpha, d_x, *beta, d_y, s);
+  (void)hipEventRecord(comp.b, s);
+  p.ngroups = groups;
+  if (st == SBLAS_OK && groups > 0) {
+    (void)hipEventRecord(merge.a, s);
+    st = launch_merge(p.d_mrows, p.d_mptr, p.d_mslots, groups, p.d_partials, 1, d_y, *alpha,
+                      *beta, s);
+    (void)hipEventRecord(merge.b, s);
+  }
+  if (st != SBLAS_OK) return nan_ret();
+  (void)hipEventRecord(down.a, s);
+  if (m > 0 && hipMemcpyAsync(y, d_y, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "download failed");
+    return nan_ret();
+  }
+  (void)hipEventRecord(down.b, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "kernel execution failed");
+    return nan_ret();
+  }
+  ret.comm_time = up.seconds() + down.seconds();
+  ret.comp_time = comp.seconds();
+  ret.merg_time = groups > 0 ? merge.seconds() : 0.0;
+  ret.numa_part_time = 0.0;
+  return ret;
+}
+
+bool check_common(int m, int n, int64_t nnz, const double *alpha, const double *beta,
+                  const void *val, const void *a0, const void *a1, const double *x,
+                  const double *y, int ngpu) {
+  clear_error();
+  if (ngpu != 1) {
+    set_error(SBLAS_ERR_NGPU, "ngpu must be 1 on this engine");
+    return false;
+  }
+  if (m < 0 || n < 0 || nnz < 0 || nnz > INT_MAX - 4096 || !alpha || !beta || !a0 ||
+      (nnz > 0 && (!val || !a1 || !x)) || (m > 0 && !y)) {
+    set_error(SBLAS_ERR_ARGS, "invalid sizes or null pointers");
+    return false;
+  }
+  return true;
+}
+
+bool check_kernel(int kernel) {
+  if (kernel < 1 || kernel > 3) {
+    set_error(SBLAS_ERR_KERNEL, "kernel must be 1, 2 or 3");
+    return false;
+  }
+  return true;
+}
+
+bool check_csr(int m, int nnz, const int *rp) {
+  if (rp[0] != 0 || rp[m] != nnz) {
+    set_error(SBLAS_ERR_ARGS, "csrRowPtr must start at 0 and end at nnz");
+    return false;
+  }
+  return true;
+}
+
+// Row-sorted COO -> row pointers (the rows the reference's coo2csr would give).
+bool coo_row_ptr(int m, int nnz, const int *ri, std::vector<int> &rp) {
+  rp.assign(static_cast<size_t>(m) + 1, 0);
+  for (int i = 0; i < nnz; i++) {
+    if (ri[i] < 0 || ri[i] >= m || (i > 0 && ri[i] < ri[i - 1])) {
+      set_error(SBLAS_ERR_ARGS, "COO row indices must be sorted and in [0, m)");
+      return false;
+    }
+    rp[ri[i] + 1]++;
+  }
+  for (int r = 0; r < m; r++) rp[r + 1] += rp[r];
+  return true;
+}
+
+// CSC -> CSR by a stable counting transpose (row-major order, columns
+// ascending within a row).
+bool csc_to_csr(int m, int n, int nnz, const double *cv, const int *cp, const int *ri,
+                std::vector<int> &rp, std::vector<int> &col, std::vector<double> &val) {
+  if (cp[0] != 0 || cp[n] != nnz) {
+    set_error(SBLAS_ERR_ARGS, "cscColPtr must start at 0 and end at nnz");
+    return false;
+  }
+  rp.assign(static_cast<size_t>(m) + 1, 0);
+  for (int i = 0; i < nnz; i++) {
+    if (ri[i] < 0 || ri[i] >= m) {
+      set_error(SBLAS_ERR_ARGS, "CSC row index out of range");
+      return false;
+    }
+    rp[ri[i] + 1]++;
+  }
+  for (int r = 0; r < m; r++) rp[r + 1] += rp[r];
+  col.resize(nnz);
+  val.resize(nnz);
+  std::vector<int> next(rp.begin(), rp.end() - 1);
+  for (int c = 0; c < n; c++)
+    for (int j = cp[c]; j < cp[c + 1]; j++) {
+      const int pos = next[ri[j]]++;
+      col[pos] = c;
+      val[pos] = cv[j];
+    }
+  return true;
+}
+
+}  // namespace
+}  // namespace sblas
+
+using namespace sblas;
+
+extern "C" {
+
+spmv_ret spMV_mgpu_baseline(int m, int n, int nnz, double *alpha, double *csrVal,
+                            int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                            double *y, int ngpu) {
+  if (!check_common(m, n, nnz, alpha, beta, csrVal, csrRowPtr, csrColIndex, x, y, ngpu) ||
+      !check_csr(m, nnz, csrRowPtr))
+    return nan_ret();
+  return run_csr(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, 1,
+                 SBLAS_PART_ROWBLOCK, 0, 1, 0.0);
+}
+
+spmv_ret spMV_mgpu_v1(int m, int n, int nnz, double *alpha, double *csrVal, int *csrRowPtr,
+                      int *csrColIndex, double *x, double *beta, double *y, int ngpu,
+                      int kernel) {
+  if (!check_common(m, n, nnz, alpha, beta, csrVal, csrRowPtr, csrColIndex, x, y, ngpu) ||
+      !check_kernel(kernel) || !check_csr(m, nnz, csrRowPtr))
+    return nan_ret();
+  return run_csr(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, kernel,
+                 SBLAS_PART_V1, 0, 1, 0.0);
+}
+
+spmv_ret spMV_mgpu_v1_numa(int m, int n, int nnz, double *alpha, double *csrVal,
+                           int *csrRowPtr, int *csrColIndex, double *x, double *beta, double *y,
+                           int ngpu, int kernel, int *numa_mapping, int part_opt, int merg_opt) {
+  (void)part_opt;  // host vs device rebase: no effect on results here
+  (void)merg_opt;  // host vs device-assisted merge: the merge is always on the device
+  if (!check_common(m, n, nnz, alpha, beta, csrVal, csrRowPtr, csrColIndex, x, y, ngpu) ||
+      !check_kernel(kernel) || !check_csr(m, nnz, csrRowPtr))
+    return nan_ret();
+  if (numa_mapping && numa_mapping[0] < 0) {
+    set_error(SBLAS_ERR_ARGS, "numa_mapping entries must be >= 0");
+    return nan_ret();
+  }
+  return run_csr(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, kernel,
+                 SBLAS_PART_V1_NUMA, 0, 1, 0.0);
+}
+
+spmv_ret spMV_mgpu_v2(int m, int n, int nnz, double *alpha, double *csrVal, int *csrRowPtr,
+                      int *csrColIndex, double *x, double *beta, double *y, int ngpu,
+                      int kernel, int nb, int copy_of_workspace) {
+  if (!check_common(m, n, nnz, alpha, beta, csrVal, csrRowPtr, csrColIndex, x, y, ngpu) ||
+      !check_kernel(kernel) || !check_csr(m, nnz, csrRowPtr))
+    return nan_ret();
+  if (nb < 1 || copy_of_workspace < 1) {
+    set_error(SBLAS_ERR_ARGS, "nb and copy_of_workspace must be >= 1");
+    return nan_ret();
+  }
+  // tasks go to copy_of_workspace streams when there are few of them;
+  // a large task count runs as one fused launch (same results)
+  const int T = sblas::v2_num_tasks(nnz, nb, copy_of_workspace);
+  const int q = (T > 0 && T <= 64) ? copy_of_workspace : 1;
+  return run_csr(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, kernel,
+                 SBLAS_PART_V2, nb, q, 0.0);
+}
+
+// x-sparse forms: entries whose x[col] == 0 are dropped first
+// (src/dspmv_mgpu_v1.cu:19-74, src/dspmv_mgpu_v2.cu:35-101).
+static bool drop_zero_x(int m, int nnz, const double *val, const int *rp, const int *col,
+                        const double *x, std::vector<double> &v2, std::vector<int> &rp2,
+                        std::vector<int> &c2) {
+  (void)nnz;
+  rp2.assign(static_cast<size_t>(m) + 1, 0);
+  v2.clear();
+  c2.clear();
+  for (int i = 0; i < m; i++) {
+    for (int j = rp[i]; j < rp[i + 1]; j++)
+      if (x[col[j]] != 0.0) {
+        v2.push_back(val[j]);
+        c2.push_back(col[j]);
+      }
+    rp2[i + 1] = static_cast<int>(v2.size());
+  }
+  return true;
+}
+
+spmv_ret spMspV_mgpu_v1(int m, int n, int nnz, double *alpha, double *csrVal, int *csrRowPtr,
+                        int *csrColIndex, double *x, double *beta, double *y, int ngpu,
+                        int kernel) {
+  if (!check_common(m, n, nnz, alpha, beta, csrVal, csrRowPtr, csrColIndex, x, y, ngpu) ||
+      !check_kernel(kernel) || !check_csr(m, nnz, csrRowPtr))
+    return nan_ret();
+  const double t0 = now_s();
+  std::vector<double> v2;
+  std::vector<int> rp2, c2;
+  drop_zero_x(m, nnz, csrVal, csrRowPtr, csrColIndex, x, v2, rp2, c2);
+  return run_csr(m, n, static_cast<int>(v2.size()), alpha, v2.data(), rp2.data(), c2.data(), x,
+                 beta, y, kernel, SBLAS_PART_V1, 0, 1, now_s() - t0);
+}
+
+spmv_ret spMspV_mgpu_v1_numa(int m, int n, int nnz, double *alpha, double *csrVal,
+                             int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                             double *y, int ngpu, int kernel) {
+  (void)m; (void)n; (void)nnz; (void)alpha; (void)csrVal; (void)csrRowPtr; (void)csrColIndex;
+  (void)x; (void)beta; (void)y; (void)ngpu; (void)kernel;
+  clear_error();
+  set_error(SBLAS_ERR_UNSUPPORTED,
+            "spMspV_mgpu_v1_numa has no definition in the reference (include/spmv_kernel.h:282)");
+  return nan_ret();
+}
+
+spmv_ret spMspV_mgpu_v2(int m, int n, int nnz, double *alpha, double *csrVal, int *csrRowPtr,
+                        int *csrColIndex, double *x, double *beta, double *y, int ngpu,
+                        int kernel, int nb, int copy_of_workspace) {
+  if (!check_common(m, n, nnz, alpha, beta, csrVal, csrRowPtr, csrColIndex, x, y, ngpu) ||
+      !check_kernel(kernel) || !check_csr(m, nnz, csrRowPtr))
+    return nan_ret();
+  if (nb < 1 || copy_of_workspace < 1) {
+    set_error(SBLAS_ERR_ARGS, "nb and copy_of_workspace must be >= 1");
+    return nan_ret();
+  }
+  const double t0 = now_s();
+  std::vector<double> v2;
+  std::vector<int> rp2, c2;
+  drop_zero_x(m, nnz, csrVal, csrRowPtr, csrColIndex, x, v2, rp2, c2);
+  const int nz = static_cast<int>(v2.size());
+  const int T = sblas::v2_num_tasks(nz, nb, copy_of_workspace);
+  const int q = (T > 0 && T <= 64) ? copy_of_workspace : 1;
+  return run_csr(m, n, nz, alpha, v2.data(), rp2.data(), c2.data(), x, beta, y, kernel,
+                 SBLAS_PART_V2, nb, q, now_s() - t0);
+}
+
+spmv_ret spMV_mgpu_baseline_coo(int m, int n, int nnz, double *alpha, double *cooVal,
+                                int *cooRowIdx, int *cooColIdx, double *x, double *beta,
+                                double *y, int ngpu) {
+  if (!check_common(m, n, nnz, alpha, beta, cooVal, cooRowIdx, cooColIdx, x, y, ngpu))
+    return nan_ret();
+  const double t0 = now_s();
+  std::vector<int> rp;
+  if (!coo_row_ptr(m, nnz, cooRowIdx, rp)) return nan_ret();
+  return run_csr(m, n, nnz, alpha, cooVal, rp.data(), cooColIdx, x, beta, y, 1,
+                 SBLAS_PART_ROWBLOCK, 0, 1, now_s() - t0);
+}
+
+spmv_ret spMV_mgpu_v1_numa_coo(int m, int n, int nnz, double *alpha, double *cooVal,
+                               int *cooRowIdx, int *cooColIdx, double *x, double *beta,
+                               double *y, int ngpu, int kernel, int *numa_mapping, int part_opt,
+                               int merg_opt) {
+  (void)numa_mapping; (void)part_opt; (void)merg_opt;
+  if (!check_common(m, n, nnz, alpha, beta, cooVal, cooRowIdx, cooColIdx, x, y, ngpu) ||
+      !check_kernel(kernel))
+    return nan_ret();
+  const double t0 = now_s();
+  std::vector<int> rp;
+  if (!coo_row_ptr(m, nnz, cooRowIdx, rp)) return nan_ret();
+  return run_csr(m, n, nnz, alpha, cooVal, rp.data(), cooColIdx, x, beta, y, kernel,
+                 SBLAS_PART_V1_NUMA, 0, 1, now_s() - t0);
+}
+
+spmv_ret spMV_mgpu_baseline_csc(int m, int n, int nnz, double *alpha, double *cscVal,
+                                int *cscColPtr, int *cscRowIdx, double *x, double *beta,
+                                double *y, int ngpu) {
+  if (!check_common(m, n, nnz, alpha, beta, cscVal, cscColPtr, cscRowIdx, x, y, ngpu))
+    return nan_ret();
+  const double t0 = now_s();
+  std::vector<int> rp, col;
+  std::vector<double> val;
+  if (!csc_to_csr(m, n, nnz, cscVal, cscColPtr, cscRowIdx, rp, col, val)) return nan_ret();
+  return run_csr(m, n, nnz, alpha, val.data(), rp.data(), col.data(), x, beta, y, 1,
+                 SBLAS_PART_V1, 0, 1, now_s() - t0);
+}
+
+spmv_ret spMV_mgpu_v1_numa_csc(int m, int n, long long nnz, double *alpha, double *cscVal,
+                               int *cscColPtr, int *cscRowIndex, double *x, double *beta,
+                               double *y, int ngpu, int kernel, int *numa_mapping, int part_opt,
+                               int merg_opt) {
+  (void)numa_mapping; (void)part_opt; (void)merg_opt;
+  if (!check_common(m, n, nnz, alpha, beta, cscVal, cscColPtr, cscRowIndex, x, y, ngpu) ||
+      !check_kernel(kernel))
+    return nan_ret();
+  const double t0 = now_s();
+  std::vector<int> rp, col;
+  std::vector<double> val;
+  if (!csc_to_csr(m, n, static_cast<int>(nnz), cscVal, cscColPtr, cscRowIndex, rp, col, val))
+    return nan_ret();
+  return run_csr(m, n, static_cast<int>(nnz), alpha, val.data(), rp.data(), col.data(), x, beta,
+                 y, kernel, SBLAS_PART_V1, 0, 1, now_s() - t0);
+}
+
+// ---- searches and partitions ---------------------------------------------------
+
+int get_row_from_index(int n, int *a, int idx) {
+  return sblas::search_row_of(n, a, static_cast<int64_t>(n) + 1, idx);
+}
+int findFirstInSorted(int *a, int n, int key) {
+  return sblas::search_first(a, n, static_cast<int64_t>(n), key);
+}
+int findLastInSorted(int *a, int n, int key) {
+  return sblas::search_last(a, n, static_cast<int64_t>(n), key);
+}
+
+int sblas_partition_rowblock(int m, const int *row_ptr, int C, sblas_chunk_desc *out) {
+  return sblas::partition_rowblock(m, row_ptr, C, out);
+}
+int sblas_partition_baseline_coo(int m, int nnz, const int *row_idx, int C,
+                                 sblas_chunk_desc *out) {
+  return sblas::partition_baseline_coo(m, nnz, row_idx, C, out);
+}
+int sblas_partition_baseline_csc(int n, const int *col_ptr, int C, sblas_chunk_desc *out) {
+  return sblas::partition_rowblock(n, col_ptr, C, out);
+}
+int sblas_partition_v1(int m, const int *row_ptr, int nnz, int C, sblas_chunk_desc *out) {
+  return sblas::partition_v1(m, row_ptr, nnz, C, out);
+}
+int sblas_partition_v1_numa(int m, const int *row_ptr, int nnz, int C, const int *numa_mapping,
+                            sblas_chunk_desc *out, sblas_chunk_desc *node_out) {
+  return sblas::partition_v1_numa(m, row_ptr, nnz, C, numa_mapping, out, node_out);
+}
+int sblas_partition_v1_numa_coo(int m, int nnz, const int *row_idx, int C,
+                                const int *numa_mapping, sblas_chunk_desc *out,
+                                sblas_chunk_desc *node_out) {
+  return sblas::partition_v1_numa_coo(m, nnz, row_idx, C, numa_mapping, out, node_out);
+}
+int sblas_partition_v1_numa_csc(int n, const int *col_ptr, long long nnz, int C,
+                                const int *numa_mapping, sblas_chunk_desc *out,
+                                sblas_chunk_desc *node_out) {
+  return sblas::partition_v1_numa_csc(n, col_ptr, nnz, C, numa_mapping, out, node_out);
+}
+int sblas_v2_num_tasks(int nnz, int nb, int q) { return sblas::v2_num_tasks(nnz, nb, q); }
+int sblas_partition_v2(int m, const int *row_ptr, int nnz, int nb, int q, sblas_chunk_desc *out,
+                       int out_cap) {
+  return sblas::partition_v2(m, row_ptr, nnz, nb, q, out, out_cap);
+}
+
+}  // extern "C"

@@ -1,0 +1,562 @@
+// plan.cpp -- analysis and execution of SpMV plans on device-resident arrays.
+//
+// A plan is the engine's counterpart of the reference's per-call work
+// (partition src/dspmv_mgpu_v1.cu:119-193, kernel switch :256-278, merge
+// :304-319) and of the CSR5 handle (include/anonymouslib_cuda.h:11-53): it
+// cuts the matrix into chunks with one of the reference's partition rules,
+// picks a kernel and its launch shape from row-length statistics, uploads the
+// chunk table and split-row merge lists once, and then runs
+// y = alpha*A*x + beta*y on a caller stream as often as asked.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+namespace sblas {
+
+namespace {
+thread_local int g_last_error = SBLAS_OK;
+thread_local std::string g_last_msg;
+
+int pow2_round(double v, int lo, int hi) {
+  if (!(v > 0)) return lo;
+  const int p = 1 << static_cast<int>(std::lround(std::log2(v)));
+  return std::min(std::max(p, lo), hi);
+}
+
+int env_int(const char *name, int fallback) {
+  const char *e = std::getenv(name);
+  return (e && *e) ? std::atoi(e) : fallback;
+}
+}  // namespace
+
+void set_error(int status, const std::string &what) {
+  g_last_error = status;
+  g_last_msg = what;
+  static const bool verbose = [] {
+    const char *e = std::getenv("SBLAS_LOG");
+    return e && std::atoi(e) > 0;
+  }();
+  if (verbose && status != SBLAS_OK) std::fprintf(stderr, "sblas: %s\n", what.c_str());
+}
+int last_error() { return g_last_error; }
+const char *last_message() { return g_last_msg.c_str(); }
+void clear_error() {
+  g_last_error = SBLAS_OK;
+  g_last_msg.clear();
+}
+
+RowStats row_stats(int m, const int *rp) {
+  RowStats s;
+  if (m <= 0) return s;
+  double sum = 0, sum2 = 0;
+  int64_t mx = 0, empty = 0;
+  for (int r = 0; r < m; r++) {
+    const int64_t L = static_cast<int64_t>(rp[r + 1]) - rp[r];
+    sum += static_cast<double>(L);
+    sum2 += static_cast<double>(L) * static_cast<double>(L);
+    mx = std::max(mx, L);
+    empty += (L == 0);
+  }
+  s.mean = sum / m;
+  const double var = std::max(0.0, sum2 / m - s.mean * s.mean);
+  s.cv = s.mean > 0 ? std::sqrt(var) / s.mean : 0.0;
+  s.max_row = mx;
+  s.empty_frac = static_cast<double>(empty) / m;
+  return s;
+}
+
+// Kernel choice from row-length statistics: rows of similar length go to the
+// vector kernel (one sub-wave per row); heavy-tailed rows, where one row can
+// be thousands of times the mean, go to the merge path, whose work per block
+// is bounded whatever the row lengths.
+int autotune_kernel(const RowStats &s) {
+  const bool heavy =
+      (s.cv > 1.5) || (s.max_row > 64 * std::max(1.0, s.mean) && s.max_row > 4096);
+  return heavy ? SBLAS_KERNEL_MERGE : SBLAS_KERNEL_VECTOR;
+}
+
+// Lanes per row: about three quarters of the mean row length, so most rows
+// finish in one or two strides of the sub-wave.
+int autotune_subwave(const RowStats &s) { return pow2_round(s.mean * 0.75, 2, 64); }
+
+// CSR5-style tile depth: the reference's rule (include/anonymouslib_cuda.h:
+// 294-318) gives sigma in [4, 32] from nnz/m; tiles here are 64 lanes wide,
+// and sigma is kept in [8, 32] so each lane has several elements in flight.
+int autotune_sigma(const RowStats &s) {
+  const int per_row = static_cast<int>(s.mean);
+  int sigma = per_row <= 4 ? 4 : (per_row <= 32 ? per_row : (per_row <= 256 ? 32 : 6));
+  return std::max(8, std::min(32, sigma));
+}
+
+// ---------------------------------------------------------------------------
+
+Plan::~Plan() {
+  for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+  for (void *p : device_buffers) (void)hipFree(p);
+  for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
+}
+
+void *Plan::dev_alloc(size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  device_buffers.push_back(p);
+  workspace_bytes += static_cast<int64_t>(bytes);
+  return p;
+}
+
+namespace {
+
+// Vector-kernel launch shape over a set of execution chunks: a block owns
+// rows_per_block consecutive rows of one chunk.
+VectorLaunch vector_shape(const std::vector<ExecChunk> &ex, double mean, int subwave,
+                          int xcd_remap, std::vector<DevChunk> &dev) {
+  VectorLaunch L{};
+  L.subwave = subwave;
+  const int subs = 256 / subwave;
+  int64_t rows_total = 0;
+  for (const ExecChunk &e : ex) rows_total += e.row_end - e.row_begin;
+  // about 4K elements per block, but keep >= ~2048 blocks when rows allow
+  int per_sub = static_cast<int>(std::max(1.0, 4096.0 / (subs * std::max(1.0, mean))));
+  const int64_t cap = std::max<int64_t>(1, rows_total / (static_cast<int64_t>(subs) * 2048));
+  per_sub = static_cast<int>(std::min<int64_t>(per_sub, cap));
+  L.rows_per_block = subs * std::max(1, per_sub);
+  dev.clear();
+  int blk = 0;
+  for (const ExecChunk &e : ex) {
+    DevChunk d{};
+    d.lo = static_cast<int>(e.lo);
+    d.hi = static_cast<int>(e.hi);
+    d.row_begin = e.row_begin;
+    d.row_end = e.row_end;
+    d.blk_begin = blk;
+    blk += (e.row_end - e.row_begin + L.rows_per_block - 1) / L.rows_per_block;
+    dev.push_back(d);
+  }
+  L.total_blocks = blk;
+  L.xcd_remap = xcd_remap;
+  L.grid = xcd_remap ? ((blk + 7) / 8) * 8 : blk;
+  return L;
+}
+
+// Split-row merge lists: every row that more than one chunk touches, with the
+// partial slots of those chunks in chunk order (slot 2c = chunk c's first
+// row, 2c+1 = its last row).
+void merge_lists(const std::vector<ExecChunk> &ex, const int *rp, std::vector<int> &rows,
+                 std::vector<int> &ptr, std::vector<int> &slots) {
+  rows.clear();
+  ptr.assign(1, 0);
+  slots.clear();
+  for (size_t c = 0; c < ex.size(); c++) {
+    const ExecChunk &e = ex[c];
+    if (e.row_end <= e.row_begin) continue;
+    const int first = e.row_begin, last = e.row_end - 1;
+    auto split = [&](int r) { return rp[r] < e.lo || rp[r + 1] > e.hi; };
+    auto add = [&](int r, int slot) {
+      if (!rows.empty() && rows.back() == r) {
+        slots.push_back(slot);
+        ptr.back() = static_cast<int>(slots.size());
+      } else {
+        rows.push_back(r);
+        slots.push_back(slot);
+        ptr.push_back(static_cast<int>(slots.size()));
+      }
+    };
+    if (split(first)) add(first, static_cast<int>(2 * c));
+    if (last != first && split(last)) add(last, static_cast<int>(2 * c + 1));
+  }
+}
+
+template <typename T>
+int upload(Plan &p, T **dst, const std::vector<T> &src) {
+  *dst = static_cast<T *>(p.dev_alloc(sizeof(T) * std::max<size_t>(src.size(), 1)));
+  if (!*dst) {
+    set_error(SBLAS_ERR_ALLOC, "plan: device allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  if (!src.empty())
+    SBLAS_HIP_TRY(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return SBLAS_OK;
+}
+
+}  // namespace
+
+int Plan::build_csr_chunks(const int *h_rp) {
+  const int C = chunks;
+  const int inz = static_cast<int>(nnz);
+  const int q = std::max(opts.streams, 1);
+  desc_status = SBLAS_OK;
+  switch (opts.part_mode) {
+    case SBLAS_PART_V1_NUMA:
+      descs.assign(C, sblas_chunk_desc{});
+      desc_status = nnz > 0 ? partition_v1_numa(m, h_rp, inz, C, nullptr, descs.data(), nullptr)
+                            : SBLAS_ERR_UNDEFINED;
+      break;
+    case SBLAS_PART_V2: {
+      const int T = v2_num_tasks(inz, std::max(opts.nb, 1), q);
+      if (T > 0) {
+        descs.assign(T, sblas_chunk_desc{});
+        const int got = partition_v2(m, h_rp, inz, std::max(opts.nb, 1), q, descs.data(), T);
+        desc_status = got > 0 ? SBLAS_OK : -got;
+      } else {
+        descs.clear();
+        desc_status = T == 0 ? SBLAS_OK : -T;
+      }
+      break;
+    }
+    case SBLAS_PART_ROWBLOCK:
+      descs.assign(C, sblas_chunk_desc{});
+      desc_status = partition_rowblock(m, h_rp, C, descs.data());
+      break;
+    case SBLAS_PART_V1:
+    default:
+      descs.assign(C, sblas_chunk_desc{});
+      desc_status = partition_v1(m, h_rp, inz, C, descs.data());
+      break;
+  }
+  int nb_eff = 1;
+  if (opts.part_mode == SBLAS_PART_V2) {
+    const int cap = static_cast<int>(0.8 * 16 * 1e9 / 16.0) / q;
+    nb_eff = std::max(1, std::min(std::max(opts.nb, 1), cap));
+  }
+  exec = exec_chunks_from_bounds(m, h_rp, element_bounds(opts.part_mode, m, h_rp, nnz, C, nb_eff));
+  std::vector<int> mrows, mptr, mslots;
+  merge_lists(exec, h_rp, mrows, mptr, mslots);
+  ngroups = static_cast<int>(mrows.size());
+  split_rows = ngroups;
+  int st;
+  if ((st = upload(*this, &d_mrows, mrows)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_mptr, mptr)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_mslots, mslots)) != SBLAS_OK) return st;
+  d_partials = static_cast<double *>(dev_alloc(sizeof(double) * 2 * exec.size() * max_k));
+  if (!d_partials) {
+    set_error(SBLAS_ERR_ALLOC, "plan: partials allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  return SBLAS_OK;
+}
+
+int Plan::setup_vector(const int *h_rp) {
+  (void)h_rp;
+  int sub = opts.subwave > 0 ? opts.subwave : autotune_subwave(stats);
+  if (sub < 2 || sub > 64 || (sub & (sub - 1))) {
+    set_error(SBLAS_ERR_ARGS, "subwave must be a power of two in [2, 64]");
+    return SBLAS_ERR_ARGS;
+  }
+  int remap = opts.xcd_remap;
+  if (remap < 0) remap = env_int("SBLAS_XCD_REMAP", 1);
+  vl = vector_shape(exec, stats.mean, sub, remap ? 1 : 0, dev_chunks);
+  int st;
+  if ((st = upload(*this, &d_chunks, dev_chunks)) != SBLAS_OK) return st;
+  if (opts.streams > 1 && exec.size() > 1) {
+    // one launch per chunk: per-chunk tables with blk_begin = 0
+    std::vector<DevChunk> solo;
+    solo_vl.clear();
+    for (size_t c = 0; c < exec.size(); c++) {
+      std::vector<ExecChunk> one(1, exec[c]);
+      std::vector<DevChunk> d1;
+      solo_vl.push_back(vector_shape(one, stats.mean, sub, remap ? 1 : 0, d1));
+      solo.push_back(d1[0]);
+    }
+    if ((st = upload(*this, &d_chunks_solo, solo)) != SBLAS_OK) return st;
+    for (int i = 0; i < opts.streams; i++) {
+      hipStream_t s;
+      SBLAS_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      streams.push_back(s);
+    }
+    for (size_t i = 0; i < exec.size() + 1; i++) {
+      hipEvent_t e;
+      SBLAS_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      events.push_back(e);
+    }
+  }
+  return SBLAS_OK;
+}
+
+hipEvent_t *Plan::timing_slot() {
+  if (!timing) return nullptr;
+  const int cap = 4096;
+  if (tcount >= cap) return nullptr;
+  while (static_cast<int>(tev.size()) < 2 * (tcount + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    tev.push_back(e);
+  }
+  return &tev[2 * tcount++];
+}
+
+int Plan::run_spmv(double alpha, const double *x, double beta, double *y, hipStream_t s) {
+  return run_spmm(1, alpha, x, beta, y, s);
+}
+
+int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
+                   hipStream_t s) {
+  launches = 0;
+  if (m == 0) return SBLAS_OK;
+  if (k < 1 || k > max_k) {
+    set_error(SBLAS_ERR_ARGS, "spmm: k out of range");
+    return SBLAS_ERR_ARGS;
+  }
+  int st = SBLAS_OK;
+  const int C = static_cast<int>(exec.size());
+  if (!streams.empty()) {
+    // chunks over streams; the caller stream waits for all of them
+    SBLAS_HIP_TRY(hipEventRecord(events[C], s));
+    for (int c = 0; c < C; c++) {
+      hipStream_t cs = streams[c % streams.size()];
+      SBLAS_HIP_TRY(hipStreamWaitEvent(cs, events[C], 0));
+      if (k == 1)
+        st = launch_csr_vector(val_type, solo_vl[c], d_chunks_solo + c, 1, a0, a1, val, X, Y,
+                               alpha, beta, d_partials + 2 * c, cs);
+      else
+        st = launch_csr_vector_mrhs(val_type, k, solo_vl[c], d_chunks_solo + c, 1, a0, a1, val,
+                                    X, Y, alpha, beta, d_partials + 2 * c * k, cs);
+      if (st != SBLAS_OK) return st;
+      SBLAS_HIP_TRY(hipEventRecord(events[c], cs));
+      launches++;
+    }
+    for (int c = 0; c < C; c++) SBLAS_HIP_TRY(hipStreamWaitEvent(s, events[c], 0));
+  } else {
+    hipEvent_t *te = timing_slot();
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
+    if (k == 1)
+      st = launch_csr_vector(val_type, vl, d_chunks, C, a0, a1, val, X, Y, alpha, beta,
+                             d_partials, s);
+    else
+      st = launch_csr_vector_mrhs(val_type, k, vl, d_chunks, C, a0, a1, val, X, Y, alpha,
+                                  beta, d_partials, s);
+    if (st != SBLAS_OK) return st;
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
+    launches++;
+  }
+  if (ngroups > 0) {
+    st = launch_merge(d_mrows, d_mptr, d_mslots, ngroups, d_partials, k, Y, alpha, beta, s);
+    if (st != SBLAS_OK) return st;
+    launches++;
+  }
+  return SBLAS_OK;
+}
+
+}  // namespace sblas
+
+// ===========================================================================
+// C API
+// ===========================================================================
+using sblas::Plan;
+
+extern "C" {
+
+int sblas_last_error(void) { return sblas::last_error(); }
+
+const char *sblas_error_string(int status) {
+  switch (status) {
+    case SBLAS_OK: return "ok";
+    case SBLAS_ERR_ARGS: return "invalid arguments";
+    case SBLAS_ERR_NGPU: return "ngpu must be 1";
+    case SBLAS_ERR_KERNEL: return "kernel must be 1, 2 or 3";
+    case SBLAS_ERR_UNDEFINED: return "reference arithmetic undefined for this input";
+    case SBLAS_ERR_HIP: return "HIP runtime error";
+    case SBLAS_ERR_ALLOC: return "allocation failed";
+    case SBLAS_ERR_UNSUPPORTED: return "unsupported entry point";
+    case SBLAS_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+  }
+}
+
+const char *sblas_version(void) { return "sblas-mi355x 0.1 (gfx950)"; }
+
+void sblas_opts_default(sblas_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->kernel = SBLAS_KERNEL_AUTO;
+  o->chunks = 0;
+  o->part_mode = SBLAS_PART_V1;
+  o->nb = 0;
+  o->streams = 1;
+  o->subwave = 0;
+  o->sigma = 0;
+  o->xcd_remap = -1;
+}
+
+static int plan_common(int format, int m, int n, int64_t nnz, const int *a0, const int *a1,
+                       const void *val, int val_type, const sblas_opts *opts,
+                       sblas_plan *out) {
+  sblas::clear_error();
+  if (!out) {
+    sblas::set_error(SBLAS_ERR_ARGS, "plan: null output handle");
+    return SBLAS_ERR_ARGS;
+  }
+  *out = nullptr;
+  if (m < 0 || n < 0 || nnz < 0 || nnz > INT_MAX - 4096 || !a0 || (nnz > 0 && (!a1 || !val)) ||
+      val_type < SBLAS_VAL_F64 || val_type > SBLAS_VAL_BF16) {
+    sblas::set_error(SBLAS_ERR_ARGS, "plan: invalid sizes, pointers or value type");
+    return SBLAS_ERR_ARGS;
+  }
+  sblas_plan h = new (std::nothrow) sblas_plan_s;
+  if (!h) {
+    sblas::set_error(SBLAS_ERR_ALLOC, "plan: host allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  Plan &p = h->impl;
+  p.format = format;
+  p.val_type = val_type;
+  p.m = m;
+  p.n = n;
+  p.nnz = nnz;
+  p.a0 = a0;
+  p.a1 = a1;
+  p.val = val;
+  if (opts) p.opts = *opts; else sblas_opts_default(&p.opts);
+  if (hipGetDevice(&p.device) != hipSuccess) {
+    delete h;
+    sblas::set_error(SBLAS_ERR_NO_DEVICE, "plan: no HIP device");
+    return SBLAS_ERR_NO_DEVICE;
+  }
+  int C = p.opts.chunks > 0 ? p.opts.chunks : sblas::env_int("SBLAS_CHUNKS", 1);
+  p.chunks = std::max(1, C);
+  *out = h;
+  return SBLAS_OK;
+}
+
+int sblas_plan_create_csr(int m, int n, int64_t nnz, const int *d_row_ptr, const int *d_col,
+                          const void *d_val, int val_type, const sblas_opts *opts,
+                          sblas_plan *plan) {
+  int st = plan_common(SBLAS_FMT_CSR, m, n, nnz, d_row_ptr, d_col, d_val, val_type, opts, plan);
+  if (st != SBLAS_OK) return st;
+  Plan &p = (*plan)->impl;
+  std::vector<int> h_rp(static_cast<size_t>(m) + 1);
+  if (hipMemcpy(h_rp.data(), d_row_ptr, sizeof(int) * h_rp.size(), hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    sblas::set_error(SBLAS_ERR_HIP, "plan: row pointer download failed");
+    delete *plan;
+    *plan = nullptr;
+    return SBLAS_ERR_HIP;
+  }
+  if (h_rp[0] != 0 || h_rp[m] != nnz) {
+    sblas::set_error(SBLAS_ERR_ARGS, "plan: row_ptr[0] must be 0 and row_ptr[m] == nnz");
+    delete *plan;
+    *plan = nullptr;
+    return SBLAS_ERR_ARGS;
+  }
+  p.stats = sblas::row_stats(m, h_rp.data());
+  p.kernel = p.opts.kernel != SBLAS_KERNEL_AUTO ? p.opts.kernel : sblas::autotune_kernel(p.stats);
+  if (p.kernel == SBLAS_KERNEL_MERGE || p.kernel == SBLAS_KERNEL_CSR5) {
+    // until the tile kernels land every CSR plan runs the vector kernel
+    p.kernel = SBLAS_KERNEL_VECTOR;
+  }
+  if ((st = p.build_csr_chunks(h_rp.data())) != SBLAS_OK ||
+      (st = p.setup_vector(h_rp.data())) != SBLAS_OK) {
+    delete *plan;
+    *plan = nullptr;
+    return st;
+  }
+  return SBLAS_OK;
+}
+
+int sblas_plan_create_coo(int m, int n, int64_t nnz, const int *d_row_idx, const int *d_col_idx,
+                          const void *d_val, int val_type, const sblas_opts *opts,
+                          sblas_plan *plan) {
+  (void)m; (void)n; (void)nnz; (void)d_row_idx; (void)d_col_idx; (void)d_val; (void)val_type;
+  (void)opts;
+  if (plan) *plan = nullptr;
+  sblas::set_error(SBLAS_ERR_UNSUPPORTED, "COO plans not built yet");
+  return SBLAS_ERR_UNSUPPORTED;
+}
+
+int sblas_plan_create_csc(int m, int n, int64_t nnz, const int *d_col_ptr, const int *d_row_idx,
+                          const void *d_val, int val_type, const sblas_opts *opts,
+                          sblas_plan *plan) {
+  (void)m; (void)n; (void)nnz; (void)d_col_ptr; (void)d_row_idx; (void)d_val; (void)val_type;
+  (void)opts;
+  if (plan) *plan = nullptr;
+  sblas::set_error(SBLAS_ERR_UNSUPPORTED, "CSC plans not built yet");
+  return SBLAS_ERR_UNSUPPORTED;
+}
+
+int sblas_plan_spmv(sblas_plan plan, double alpha, const double *d_x, double beta, double *d_y,
+                    void *stream) {
+  sblas::clear_error();
+  if (!plan || (!d_y && plan->impl.m > 0) || (!d_x && plan->impl.nnz > 0)) {
+    sblas::set_error(SBLAS_ERR_ARGS, "spmv: null plan or vector");
+    return SBLAS_ERR_ARGS;
+  }
+  return plan->impl.run_spmv(alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream));
+}
+
+int sblas_plan_spmm(sblas_plan plan, int k, double alpha, const double *d_X, double beta,
+                    double *d_Y, void *stream) {
+  sblas::clear_error();
+  if (!plan || (!d_Y && plan->impl.m > 0) || (!d_X && plan->impl.nnz > 0)) {
+    sblas::set_error(SBLAS_ERR_ARGS, "spmm: null plan or block");
+    return SBLAS_ERR_ARGS;
+  }
+  return plan->impl.run_spmm(k, alpha, d_X, beta, d_Y, static_cast<hipStream_t>(stream));
+}
+
+int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info) {
+  if (!plan || !info) return SBLAS_ERR_ARGS;
+  const Plan &p = plan->impl;
+  std::memset(info, 0, sizeof(*info));
+  info->format = p.format;
+  info->val_type = p.val_type;
+  info->kernel = p.kernel;
+  info->chunks = static_cast<int>(p.exec.size());
+  info->subwave = p.vl.subwave;
+  info->sigma = p.sigma;
+  info->xcd_remap = p.vl.xcd_remap;
+  info->split_rows = p.split_rows;
+  info->m = p.m;
+  info->n = p.n;
+  info->nnz = p.nnz;
+  info->workspace_bytes = p.workspace_bytes;
+  info->mean_row = p.stats.mean;
+  info->cv_row = p.stats.cv;
+  info->max_row = p.stats.max_row;
+  info->launches = p.launches;
+  return SBLAS_OK;
+}
+
+int sblas_plan_get_chunks(sblas_plan plan, sblas_chunk_desc *out, int cap) {
+  if (!plan) return -SBLAS_ERR_ARGS;
+  const Plan &p = plan->impl;
+  if (p.desc_status != SBLAS_OK) return -p.desc_status;
+  const int n = static_cast<int>(p.descs.size());
+  if (out) std::memcpy(out, p.descs.data(), sizeof(sblas_chunk_desc) * std::min(n, cap));
+  return n;
+}
+
+int sblas_plan_set_timing(sblas_plan plan, int on) {
+  if (!plan) return SBLAS_ERR_ARGS;
+  plan->impl.timing = on != 0;
+  plan->impl.tcount = 0;
+  return SBLAS_OK;
+}
+
+int sblas_plan_kernel_times(sblas_plan plan, float *ms, int cap) {
+  if (!plan) return -SBLAS_ERR_ARGS;
+  Plan &p = plan->impl;
+  const int n = std::min(p.tcount, std::max(cap, 0));
+  for (int i = 0; i < n; i++) {
+    if (hipEventSynchronize(p.tev[2 * i + 1]) != hipSuccess) return -SBLAS_ERR_HIP;
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, p.tev[2 * i], p.tev[2 * i + 1]) != hipSuccess)
+      return -SBLAS_ERR_HIP;
+    if (ms) ms[i] = t;
+  }
+  p.tcount = 0;
+  return n;
+}
+
+int sblas_plan_destroy(sblas_plan plan) {
+  delete plan;
+  return SBLAS_OK;
+}
+
+}  // extern "C"

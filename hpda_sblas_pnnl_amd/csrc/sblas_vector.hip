@@ -1,0 +1,308 @@
+// sblas_vector.hip -- vector-CSR SpMV / SpMM for gfx950, the split-row merge
+// and small helpers.
+//
+// Replaces the reference's kernel 1 (cusparseDcsrmv, call sites
+// src/dspmv_mgpu_v1.cu:261, src/dspmv_mgpu_baseline.cu:125,
+// src/dspmv_mgpu_v1_numa.cu:696) and its host merges
+// (src/dspmv_mgpu_v1.cu:304-319, src/dspmv_mgpu_v2.cu:486-544).
+//
+// Design: a 64-lane wave is cut into sub-waves of SUBW lanes, one row per
+// sub-wave at a time; SUBW follows the mean row length.  A launch block owns a
+// contiguous run of rows of one chunk (rows_per_block), so consecutive blocks
+// touch neighbouring parts of x; with xcd_remap the logical block order is
+// re-dealt so that each XCD (blocks are dealt round-robin over the 8 XCDs)
+// owns a contiguous 1/8 of the rows and keeps its own x window in its L2.
+// Matrix values and column indices are streamed with non-temporal loads so
+// they do not evict x from L2 / MALL.  Products and sums are FP64 for every
+// stored value type.
+#include "internal.hpp"
+
+namespace sblas {
+namespace {
+
+__device__ __forceinline__ double load_val(const double *p, int j) {
+  return __builtin_nontemporal_load(p + j);
+}
+__device__ __forceinline__ double load_val(const float *p, int j) {
+  return static_cast<double>(__builtin_nontemporal_load(p + j));
+}
+__device__ __forceinline__ double load_val(const uint16_t *p, int j) {
+  const uint32_t h = __builtin_nontemporal_load(p + j);
+  return static_cast<double>(__uint_as_float(h << 16));
+}
+__device__ __forceinline__ int load_col(const int *p, int j) {
+  return __builtin_nontemporal_load(p + j);
+}
+
+// Sum over the SUBW lanes of each aligned sub-wave (xor butterfly).
+template <int SUBW>
+__device__ __forceinline__ double subwave_sum(double v) {
+#pragma unroll
+  for (int off = SUBW / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, SUBW);
+  return v;
+}
+
+__device__ __forceinline__ int logical_block(int total_blocks, int xcd_remap) {
+  int b = blockIdx.x;
+  if (xcd_remap) {
+    const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    b = (b & 7) * per + (b >> 3);
+  }
+  return b < total_blocks ? b : -1;
+}
+
+__device__ __forceinline__ int chunk_of_block(const DevChunk *chunks, int nchunks, int b) {
+  int lo = 0, hi = nchunks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chunks[mid].blk_begin <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename VT, int SUBW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_csr_vector(
+    const DevChunk *__restrict__ chunks, int nchunks, int rows_per_block, int total_blocks,
+    int xcd_remap, const int *__restrict__ rp, const int *__restrict__ col,
+    const VT *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+    double alpha, double beta, double *__restrict__ partials) {
+  const int b = logical_block(total_blocks, xcd_remap);
+  if (b < 0) return;
+  const int c = nchunks > 1 ? chunk_of_block(chunks, nchunks, b) : 0;
+  const DevChunk ch = chunks[c];
+  constexpr int SUBS = BLOCK / SUBW;
+  const int sub = threadIdx.x / SUBW;
+  const int lane = threadIdx.x % SUBW;
+  const int row0 = ch.row_begin + (b - ch.blk_begin) * rows_per_block;
+  const int row1 = min(row0 + rows_per_block, ch.row_end);
+  for (int r = row0 + sub; r < row1; r += SUBS) {
+    const int rs = rp[r], re = rp[r + 1];
+    const int s = max(rs, ch.lo), e = min(re, ch.hi);
+    double sum = 0.0;
+    int j = s + lane;
+    // two independent accumulators keep two loads in flight per lane
+    double sum2 = 0.0;
+    for (; j + SUBW < e; j += 2 * SUBW) {
+      const double v0 = load_val(val, j);
+      const int c0 = load_col(col, j);
+      const double v1 = load_val(val, j + SUBW);
+      const int c1 = load_col(col, j + SUBW);
+      sum += v0 * x[c0];
+      sum2 += v1 * x[c1];
+    }
+    if (j < e) sum += load_val(val, j) * x[load_col(col, j)];
+    sum = subwave_sum<SUBW>(sum + sum2);
+    if (lane == 0) {
+      if (rs < ch.lo || re > ch.hi) {
+        partials[2 * c + (r == ch.row_begin ? 0 : 1)] = sum;
+      } else {
+        y[r] = (beta == 0.0) ? alpha * sum : alpha * sum + beta * y[r];
+      }
+    }
+  }
+}
+
+// k right-hand sides, X row-major (n x K), Y row-major (m x K).  Lane l of a
+// sub-wave handles vector l % K and every (SUBW/K)-th element of the row, so
+// K neighbouring lanes read one contiguous K*8-byte row of X.
+template <typename VT, int SUBW, int K, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_csr_vector_mrhs(
+    const DevChunk *__restrict__ chunks, int nchunks, int rows_per_block, int total_blocks,
+    int xcd_remap, const int *__restrict__ rp, const int *__restrict__ col,
+    const VT *__restrict__ val, const double *__restrict__ X, double *__restrict__ Y,
+    double alpha, double beta, double *__restrict__ partials) {
+  static_assert(SUBW >= K, "sub-wave narrower than K");
+  const int b = logical_block(total_blocks, xcd_remap);
+  if (b < 0) return;
+  const int c = nchunks > 1 ? chunk_of_block(chunks, nchunks, b) : 0;
+  const DevChunk ch = chunks[c];
+  constexpr int SUBS = BLOCK / SUBW;
+  constexpr int STEP = SUBW / K;
+  const int sub = threadIdx.x / SUBW;
+  const int lane = threadIdx.x % SUBW;
+  const int vec = lane % K;
+  const int grp = lane / K;
+  const int row0 = ch.row_begin + (b - ch.blk_begin) * rows_per_block;
+  const int row1 = min(row0 + rows_per_block, ch.row_end);
+  for (int r = row0 + sub; r < row1; r += SUBS) {
+    const int rs = rp[r], re = rp[r + 1];
+    const int s = max(rs, ch.lo), e = min(re, ch.hi);
+    double sum = 0.0;
+    for (int j = s + grp; j < e; j += STEP) {
+      const double v = load_val(val, j);
+      const int cc = load_col(col, j);
+      sum += v * X[static_cast<int64_t>(cc) * K + vec];
+    }
+#pragma unroll
+    for (int off = SUBW / 2; off >= K; off >>= 1) sum += __shfl_xor(sum, off, SUBW);
+    if (grp == 0) {
+      if (rs < ch.lo || re > ch.hi) {
+        partials[(2 * c + (r == ch.row_begin ? 0 : 1)) * K + vec] = sum;
+      } else {
+        double *yp = Y + static_cast<int64_t>(r) * K + vec;
+        *yp = (beta == 0.0) ? alpha * sum : alpha * sum + beta * (*yp);
+      }
+    }
+  }
+}
+
+// Split rows: y[row] = alpha * (partials of the row in chunk order) + beta*y.
+__global__ void k_merge(const int *__restrict__ rows, const int *__restrict__ ptr,
+                        const int *__restrict__ slots, int groups,
+                        const double *__restrict__ partials, int k, double *__restrict__ y,
+                        double alpha, double beta) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = t / k, v = t % k;
+  if (g >= groups) return;
+  double s = 0.0;
+  for (int q = ptr[g]; q < ptr[g + 1]; q++) s += partials[slots[q] * k + v];
+  double *yp = y + static_cast<int64_t>(rows[g]) * k + v;
+  *yp = (beta == 0.0) ? alpha * s : alpha * s + beta * (*yp);
+}
+
+__global__ void k_scale(int64_t count, double beta, double *__restrict__ y) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) y[i] = (beta == 0.0) ? 0.0 : beta * y[i];
+}
+
+template <typename VT>
+int launch_vector_typed(const VectorLaunch &L, const DevChunk *chunks, int nchunks,
+                        const int *rp, const int *col, const VT *val, const double *x,
+                        double *y, double alpha, double beta, double *partials,
+                        hipStream_t s) {
+  constexpr int B = 256;
+  const dim3 grid(L.grid), block(B);
+#define SBLAS_VEC_CASE(W)                                                                 \
+  case W:                                                                                 \
+    hipLaunchKernelGGL((k_csr_vector<VT, W, B>), grid, block, 0, s, chunks, nchunks,      \
+                       L.rows_per_block, L.total_blocks, L.xcd_remap, rp, col, val, x, y, \
+                       alpha, beta, partials);                                            \
+    break;
+  switch (L.subwave) {
+    SBLAS_VEC_CASE(2)
+    SBLAS_VEC_CASE(4)
+    SBLAS_VEC_CASE(8)
+    SBLAS_VEC_CASE(16)
+    SBLAS_VEC_CASE(32)
+    SBLAS_VEC_CASE(64)
+    default:
+      set_error(SBLAS_ERR_ARGS, "vector kernel: sub-wave must be a power of two in [2,64]");
+      return SBLAS_ERR_ARGS;
+  }
+#undef SBLAS_VEC_CASE
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+template <typename VT, int K>
+int launch_mrhs_k(const VectorLaunch &L, const DevChunk *chunks, int nchunks, const int *rp,
+                  const int *col, const VT *val, const double *X, double *Y, double alpha,
+                  double beta, double *partials, hipStream_t s) {
+  constexpr int B = 256;
+  const dim3 grid(L.grid), block(B);
+  const int w = L.subwave < K ? K : L.subwave;
+#define SBLAS_MRHS_CASE(W)                                                                 \
+  case W:                                                                                  \
+    if constexpr (W >= K)                                                                  \
+      hipLaunchKernelGGL((k_csr_vector_mrhs<VT, W, K, B>), grid, block, 0, s, chunks,      \
+                         nchunks, L.rows_per_block, L.total_blocks, L.xcd_remap, rp, col,  \
+                         val, X, Y, alpha, beta, partials);                                \
+    break;
+  switch (w) {
+    SBLAS_MRHS_CASE(2)
+    SBLAS_MRHS_CASE(4)
+    SBLAS_MRHS_CASE(8)
+    SBLAS_MRHS_CASE(16)
+    SBLAS_MRHS_CASE(32)
+    SBLAS_MRHS_CASE(64)
+    default:
+      set_error(SBLAS_ERR_ARGS, "spmm kernel: bad sub-wave");
+      return SBLAS_ERR_ARGS;
+  }
+#undef SBLAS_MRHS_CASE
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+template <typename VT>
+int launch_mrhs_typed(int k, const VectorLaunch &L, const DevChunk *chunks, int nchunks,
+                      const int *rp, const int *col, const VT *val, const double *X, double *Y,
+                      double alpha, double beta, double *partials, hipStream_t s) {
+  switch (k) {
+    case 1: return launch_mrhs_k<VT, 1>(L, chunks, nchunks, rp, col, val, X, Y, alpha, beta, partials, s);
+    case 2: return launch_mrhs_k<VT, 2>(L, chunks, nchunks, rp, col, val, X, Y, alpha, beta, partials, s);
+    case 4: return launch_mrhs_k<VT, 4>(L, chunks, nchunks, rp, col, val, X, Y, alpha, beta, partials, s);
+    case 8: return launch_mrhs_k<VT, 8>(L, chunks, nchunks, rp, col, val, X, Y, alpha, beta, partials, s);
+    case 16: return launch_mrhs_k<VT, 16>(L, chunks, nchunks, rp, col, val, X, Y, alpha, beta, partials, s);
+    default:
+      set_error(SBLAS_ERR_ARGS, "spmm: k must be 1, 2, 4, 8 or 16");
+      return SBLAS_ERR_ARGS;
+  }
+}
+
+}  // namespace
+
+int launch_csr_vector(int val_type, const VectorLaunch &L, const DevChunk *chunks, int nchunks,
+                      const int *rp, const int *col, const void *val, const double *x,
+                      double *y, double alpha, double beta, double *partials,
+                      hipStream_t s) {
+  if (L.grid <= 0) return SBLAS_OK;
+  switch (val_type) {
+    case SBLAS_VAL_F64:
+      return launch_vector_typed(L, chunks, nchunks, rp, col, static_cast<const double *>(val),
+                                 x, y, alpha, beta, partials, s);
+    case SBLAS_VAL_F32:
+      return launch_vector_typed(L, chunks, nchunks, rp, col, static_cast<const float *>(val),
+                                 x, y, alpha, beta, partials, s);
+    case SBLAS_VAL_BF16:
+      return launch_vector_typed(L, chunks, nchunks, rp, col,
+                                 static_cast<const uint16_t *>(val), x, y, alpha, beta,
+                                 partials, s);
+    default:
+      set_error(SBLAS_ERR_ARGS, "unknown value type");
+      return SBLAS_ERR_ARGS;
+  }
+}
+
+int launch_csr_vector_mrhs(int val_type, int k, const VectorLaunch &L, const DevChunk *chunks,
+                           int nchunks, const int *rp, const int *col, const void *val,
+                           const double *X, double *Y, double alpha, double beta,
+                           double *partials, hipStream_t s) {
+  if (L.grid <= 0) return SBLAS_OK;
+  switch (val_type) {
+    case SBLAS_VAL_F64:
+      return launch_mrhs_typed(k, L, chunks, nchunks, rp, col, static_cast<const double *>(val),
+                               X, Y, alpha, beta, partials, s);
+    case SBLAS_VAL_F32:
+      return launch_mrhs_typed(k, L, chunks, nchunks, rp, col, static_cast<const float *>(val),
+                               X, Y, alpha, beta, partials, s);
+    case SBLAS_VAL_BF16:
+      return launch_mrhs_typed(k, L, chunks, nchunks, rp, col,
+                               static_cast<const uint16_t *>(val), X, Y, alpha, beta,
+                               partials, s);
+    default:
+      set_error(SBLAS_ERR_ARGS, "unknown value type");
+      return SBLAS_ERR_ARGS;
+  }
+}
+
+int launch_merge(const int *rows, const int *ptr, const int *slots, int groups,
+                 const double *partials, int k, double *y, double alpha, double beta,
+                 hipStream_t s) {
+  if (groups <= 0) return SBLAS_OK;
+  const int threads = groups * k;
+  hipLaunchKernelGGL(k_merge, dim3((threads + 255) / 256), dim3(256), 0, s, rows, ptr, slots,
+                     groups, partials, k, y, alpha, beta);
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+int launch_scale(int64_t count, double beta, double *y, hipStream_t s) {
+  if (count <= 0) return SBLAS_OK;
+  hipLaunchKernelGGL(k_scale, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s,
+                     count, beta, y);
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+}  // namespace sblas

@@ -1,0 +1,128 @@
+"""Plans over device-resident matrices (torch tensors on a HIP device).
+
+``SpmvPlan`` wraps the C plan API (include/sblas.h, section 3): the analogue
+of the reference's CSR5 handle (include/anonymouslib_cuda.h:11-53) -- analyse
+once, then ``spmv``/``spmm`` on the current torch stream as often as needed.
+torch supplies device memory and streams only; every kernel is the engine's.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib
+from ._lib import Opts, PlanInfo, check, lib
+
+_VAL_TYPES = {"f64": _lib.VAL_F64, "f32": _lib.VAL_F32, "bf16": _lib.VAL_BF16}
+_KERNELS = {"auto": _lib.KERNEL_AUTO, "vector": _lib.KERNEL_VECTOR, "merge": _lib.KERNEL_MERGE,
+            "csr5": _lib.KERNEL_CSR5, 0: 0, 1: 1, 2: 2, 3: 3}
+_PARTS = {"v1": _lib.PART_V1, "v1_numa": _lib.PART_V1_NUMA, "v2": _lib.PART_V2,
+          "rowblock": _lib.PART_ROWBLOCK}
+
+
+def _torch():
+    import torch  # noqa: PLC0415  (torch is plumbing; import lazily)
+    return torch
+
+
+def val_type_of(t) -> int:
+    torch = _torch()
+    if t.dtype == torch.float64:
+        return _lib.VAL_F64
+    if t.dtype == torch.float32:
+        return _lib.VAL_F32
+    if t.dtype in (torch.bfloat16, torch.int16, torch.uint16):
+        return _lib.VAL_BF16
+    raise TypeError(f"unsupported value dtype {t.dtype}")
+
+
+def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=0, sigma=0,
+              xcd_remap=-1) -> Opts:
+    o = Opts()
+    lib().sblas_opts_default(ctypes.byref(o))
+    o.kernel = _KERNELS[kernel]
+    o.chunks = chunks
+    o.part_mode = _PARTS[part_mode] if isinstance(part_mode, str) else part_mode
+    o.nb = nb
+    o.streams = streams
+    o.subwave = subwave
+    o.sigma = sigma
+    o.xcd_remap = xcd_remap
+    return o
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class SpmvPlan:
+    """A matrix analysed for repeated y = alpha*A*x + beta*y.
+
+    fmt 'csr': (row_ptr, col, val); 'coo': (row_idx, col_idx, val) sorted by
+    row; 'csc': (col_ptr, row_idx, val).  Index tensors int32, values float64 /
+    float32 / bfloat16, all on the same HIP device.  The tensors must stay
+    alive and unchanged while the plan exists (the plan keeps references).
+    """
+
+    def __init__(self, fmt, m, n, a0, a1, val, **opts):
+        self.fmt, self.m, self.n = fmt, int(m), int(n)
+        self._keep = (a0, a1, val)
+        self.nnz = int(val.numel())
+        o = make_opts(**opts)
+        h = ctypes.c_void_p()
+        create = {"csr": lib().sblas_plan_create_csr, "coo": lib().sblas_plan_create_coo,
+                  "csc": lib().sblas_plan_create_csc}[fmt]
+        st = create(self.m, self.n, self.nnz, a0.data_ptr(), a1.data_ptr(), val.data_ptr(),
+                    val_type_of(val), ctypes.byref(o), ctypes.byref(h))
+        check(st, f"sblas_plan_create_{fmt}")
+        self._h = h
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0, stream=None):
+        st = lib().sblas_plan_spmv(self._h, float(alpha), x.data_ptr(), float(beta),
+                                   y.data_ptr(), _stream_ptr(stream))
+        check(st, "sblas_plan_spmv")
+        return y
+
+    def spmm(self, X, Y, alpha=1.0, beta=0.0, stream=None):
+        k = 1 if X.dim() == 1 else int(X.shape[1])
+        st = lib().sblas_plan_spmm(self._h, k, float(alpha), X.data_ptr(), float(beta),
+                                   Y.data_ptr(), _stream_ptr(stream))
+        check(st, "sblas_plan_spmm")
+        return Y
+
+    def info(self) -> dict:
+        inf = PlanInfo()
+        check(lib().sblas_plan_get_info(self._h, ctypes.byref(inf)), "sblas_plan_get_info")
+        return inf.as_dict()
+
+    def chunks(self):
+        import numpy as np  # noqa: PLC0415
+        n = lib().sblas_plan_get_chunks(self._h, None, 0)
+        if n < 0:
+            return -n, np.zeros(0, dtype=_lib.CHUNK_DTYPE)
+        out = np.zeros(max(n, 1), dtype=_lib.CHUNK_DTYPE)
+        lib().sblas_plan_get_chunks(self._h, out.ctypes.data, n)
+        return 0, out[:n]
+
+    def set_timing(self, on: bool = True):
+        check(lib().sblas_plan_set_timing(self._h, int(bool(on))), "sblas_plan_set_timing")
+
+    def kernel_times_ms(self, cap: int = 4096):
+        import numpy as np  # noqa: PLC0415
+        out = np.zeros(cap, dtype=np.float32)
+        n = lib().sblas_plan_kernel_times(self._h, out.ctypes.data, cap)
+        if n < 0:
+            raise _lib.SblasError(-n, "sblas_plan_kernel_times")
+        return out[:n].astype(np.float64)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().sblas_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001  (interpreter shutdown)
+            pass

@@ -1,0 +1,214 @@
+"""The reference's C entry points (include/spmv_kernel.h:211-296), callable from
+Python on host (numpy) arrays.
+
+Same names and argument order as the reference.  ``y`` is updated in place
+with y = alpha*A*x + beta*y, ``alpha``/``beta`` are Python floats (the
+reference passes pointers), and the return value is the reference's
+``spmv_ret`` timing record.  Where the reference exits the process on an error
+(``checkCudaErrors``, include/detail/cuda/helper_cuda.h:583-595) these raise
+``SblasError``; the C ABI itself returns NaN timings and a status.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import SblasError, lib
+
+__all__ = [
+    "spMV_mgpu_baseline", "spMV_mgpu_baseline_csc", "spMV_mgpu_baseline_coo", "spMV_mgpu_v1",
+    "spMV_mgpu_v1_numa", "spMV_mgpu_v1_numa_csc", "spMV_mgpu_v1_numa_coo", "spMV_mgpu_v2",
+    "spMspV_mgpu_v1", "spMspV_mgpu_v1_numa", "spMspV_mgpu_v2", "get_row_from_index",
+    "findFirstInSorted", "findLastInSorted",
+]
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _y(y):
+    if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous):
+        raise TypeError("y must be a C-contiguous float64 numpy array (updated in place)")
+    return y
+
+
+def _scal(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+def _call(name, *args):
+    ret = getattr(lib(), name)(*args)
+    if any(math.isnan(getattr(ret, f)) for f, _ in _lib.SpmvRet._fields_):
+        raise SblasError(lib().sblas_last_error(), name)
+    return ret
+
+
+def _csr_args(m, n, nnz, alpha, val, rp, col, x, beta, y):
+    val, rp, col, x, y = _f64(val), _i32(rp), _i32(col), _f64(x), _y(y)
+    keep = (val, rp, col, x, y)
+    return keep, (m, n, nnz, _scal(alpha), val.ctypes.data, rp.ctypes.data, col.ctypes.data,
+                  x.ctypes.data, _scal(beta), y.ctypes.data)
+
+
+def spMV_mgpu_baseline(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1):
+    """Row-block CSR SpMV (src/dspmv_mgpu_baseline.cu:14-202)."""
+    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
+    return _call("spMV_mgpu_baseline", *a, ngpu)
+
+
+def spMV_mgpu_baseline_csc(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIdx, x, beta, y, ngpu=1):
+    """Column-block CSC SpMV (src/dspmv_mgpu_baseline_csc.cu:14-239)."""
+    keep, a = _csr_args(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIdx, x, beta, y)
+    return _call("spMV_mgpu_baseline_csc", *a, ngpu)
+
+
+def spMV_mgpu_baseline_coo(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y, ngpu=1):
+    """Row-block COO SpMV (src/dspmv_mgpu_baseline_coo.cu:16-264)."""
+    keep, a = _csr_args(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y)
+    return _call("spMV_mgpu_baseline_coo", *a, ngpu)
+
+
+def spMV_mgpu_v1(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
+                 kernel=1):
+    """nnz-chunked CSR SpMV with kernel switch 1/2/3 (src/dspmv_mgpu_v1.cu:76-360)."""
+    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
+    return _call("spMV_mgpu_v1", *a, ngpu, kernel)
+
+
+def _mapping(numa_mapping, ngpu):
+    mp = _i32(numa_mapping if numa_mapping is not None else [0] * max(ngpu, 1))
+    return mp
+
+
+def spMV_mgpu_v1_numa(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
+                      kernel=1, numa_mapping=None, part_opt=0, merg_opt=0):
+    """Two-level nnz-chunked CSR SpMV (src/dspmv_mgpu_v1_numa.cu:98-837)."""
+    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
+    mp = _mapping(numa_mapping, ngpu)
+    return _call("spMV_mgpu_v1_numa", *a, ngpu, kernel, mp.ctypes.data, part_opt, merg_opt)
+
+
+def spMV_mgpu_v1_numa_csc(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIndex, x, beta, y, ngpu=1,
+                          kernel=1, numa_mapping=None, part_opt=0, merg_opt=0):
+    """Two-level nnz-chunked CSC SpMV (src/dspmv_mgpu_v1_numa_csc.cu:43-483)."""
+    keep, a = _csr_args(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIndex, x, beta, y)
+    mp = _mapping(numa_mapping, ngpu)
+    return _call("spMV_mgpu_v1_numa_csc", *a, ngpu, kernel, mp.ctypes.data, part_opt, merg_opt)
+
+
+def spMV_mgpu_v1_numa_coo(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y, ngpu=1,
+                          kernel=1, numa_mapping=None, part_opt=0, merg_opt=0):
+    """Two-level nnz-chunked COO SpMV (src/dspmv_mgpu_v1_numa_coo.cu:35-633)."""
+    keep, a = _csr_args(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y)
+    mp = _mapping(numa_mapping, ngpu)
+    return _call("spMV_mgpu_v1_numa_coo", *a, ngpu, kernel, mp.ctypes.data, part_opt, merg_opt)
+
+
+def spMV_mgpu_v2(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
+                 kernel=1, nb=1 << 20, copy_of_workspace=1):
+    """Task-queue CSR SpMV, nb nonzeros per task (src/dspmv_mgpu_v2.cu:103-300)."""
+    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
+    return _call("spMV_mgpu_v2", *a, ngpu, kernel, nb, copy_of_workspace)
+
+
+def spMspV_mgpu_v1(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
+                   kernel=1):
+    """v1 after dropping entries with x[col] == 0 (src/dspmv_mgpu_v1.cu:19-74)."""
+    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
+    return _call("spMspV_mgpu_v1", *a, ngpu, kernel)
+
+
+def spMspV_mgpu_v1_numa(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
+                        kernel=1):
+    """Declared by the reference (include/spmv_kernel.h:282) but never defined:
+    always raises."""
+    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
+    return _call("spMspV_mgpu_v1_numa", *a, ngpu, kernel)
+
+
+def spMspV_mgpu_v2(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
+                   kernel=1, nb=1 << 20, copy_of_workspace=1):
+    """v2 after dropping entries with x[col] == 0 (src/dspmv_mgpu_v2.cu:35-101)."""
+    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
+    return _call("spMspV_mgpu_v2", *a, ngpu, kernel, nb, copy_of_workspace)
+
+
+def get_row_from_index(n, a, idx):
+    a = _i32(a)
+    return lib().get_row_from_index(n, a.ctypes.data, idx)
+
+
+def findFirstInSorted(a, n, key):
+    a = _i32(a)
+    return lib().findFirstInSorted(a.ctypes.data, n, key)
+
+
+def findLastInSorted(a, n, key):
+    a = _i32(a)
+    return lib().findLastInSorted(a.ctypes.data, n, key)
+
+
+# --------------------------------------------------------------------------
+# partition API
+# --------------------------------------------------------------------------
+
+def _desc(n):
+    return np.zeros(max(n, 1), dtype=_lib.CHUNK_DTYPE)
+
+
+def partition(mode: str, *, C: int, m: int = 0, n: int = 0, nnz: int = 0, row_ptr=None,
+              row_idx=None, col_ptr=None, numa_mapping=None, nb: int = 0, q: int = 1):
+    """Chunk descriptors of one of the reference's rules.
+
+    mode: 'rowblock' | 'baseline_coo' | 'baseline_csc' | 'v1' | 'v1_numa' |
+    'v1_numa_coo' | 'v1_numa_csc' | 'v2'.  Returns (status, chunks[, nodes]).
+    """
+    L = lib()
+    mp = None if numa_mapping is None else _i32(numa_mapping)
+    mpp = None if mp is None else mp.ctypes.data
+    nodes = 1 if mp is None else int(mp.max()) + 1
+    if mode == "rowblock":
+        rp = _i32(row_ptr); out = _desc(C)
+        return L.sblas_partition_rowblock(m, rp.ctypes.data, C, out.ctypes.data), out[:C]
+    if mode == "baseline_coo":
+        ri = _i32(row_idx); out = _desc(C)
+        return L.sblas_partition_baseline_coo(m, len(ri), ri.ctypes.data, C, out.ctypes.data), out[:C]
+    if mode == "baseline_csc":
+        cp = _i32(col_ptr); out = _desc(C)
+        return L.sblas_partition_baseline_csc(n, cp.ctypes.data, C, out.ctypes.data), out[:C]
+    if mode == "v1":
+        rp = _i32(row_ptr); out = _desc(C)
+        return L.sblas_partition_v1(m, rp.ctypes.data, nnz, C, out.ctypes.data), out[:C]
+    if mode == "v1_numa":
+        rp = _i32(row_ptr); out, nout = _desc(C), _desc(nodes)
+        st = L.sblas_partition_v1_numa(m, rp.ctypes.data, nnz, C, mpp, out.ctypes.data,
+                                       nout.ctypes.data)
+        return st, out[:C], nout[:nodes]
+    if mode == "v1_numa_coo":
+        ri = _i32(row_idx); out, nout = _desc(C), _desc(nodes)
+        st = L.sblas_partition_v1_numa_coo(m, len(ri), ri.ctypes.data, C, mpp, out.ctypes.data,
+                                           nout.ctypes.data)
+        return st, out[:C], nout[:nodes]
+    if mode == "v1_numa_csc":
+        cp = _i32(col_ptr); out, nout = _desc(C), _desc(nodes)
+        st = L.sblas_partition_v1_numa_csc(n, cp.ctypes.data, nnz, C, mpp, out.ctypes.data,
+                                           nout.ctypes.data)
+        return st, out[:C], nout[:nodes]
+    if mode == "v2":
+        rp = _i32(row_ptr)
+        T = L.sblas_v2_num_tasks(nnz, nb, q)
+        if T < 0:
+            return -T, _desc(0)[:0]
+        out = _desc(T)
+        got = L.sblas_partition_v2(m, rp.ctypes.data, nnz, nb, q, out.ctypes.data, T)
+        return (0 if got >= 0 else -got), out[:T]
+    raise ValueError(f"unknown partition mode {mode!r}")

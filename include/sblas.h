@@ -1,0 +1,243 @@
+/*
+ * sblas.h -- C ABI of the MI355X SpMV engine (libsblas.so).
+ *
+ * Three layers:
+ *   1. Drop-in entry points with the reference's names and argument lists
+ *      (include/spmv_kernel.h:211-296 of the reference).  Host pointers,
+ *      synchronous, y = alpha*A*x + beta*y, ngpu must be 1.
+ *   2. Partition API: the reference's chunking rules evaluated at any chunk
+ *      count C, returning one descriptor per chunk.
+ *   3. Plan API on device-resident arrays (the analogue of the reference's
+ *      CSR5 handle, include/anonymouslib_cuda.h:11-53): analyse once, run
+ *      many times on a caller stream.  Also the batched multi-matrix form.
+ *
+ * All functions are thread-safe.  Errors never exit the process: the
+ * functions return a status (or NaN timings for the spmv_ret entry points)
+ * and sblas_last_error() holds the thread's last status.
+ */
+#ifndef SBLAS_H
+#define SBLAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* status                                                                  */
+/* ---------------------------------------------------------------------- */
+enum {
+  SBLAS_OK = 0,
+  SBLAS_ERR_ARGS = 1,          /* invalid sizes, pointers or options          */
+  SBLAS_ERR_NGPU = 2,          /* ngpu != 1                                   */
+  SBLAS_ERR_KERNEL = 3,        /* kernel not in {1,2,3}                       */
+  SBLAS_ERR_UNDEFINED = 4,     /* reference arithmetic undefined (overflow /  */
+                               /* out-of-range read) for this partition input  */
+  SBLAS_ERR_HIP = 5,           /* a HIP runtime call failed                   */
+  SBLAS_ERR_ALLOC = 6,         /* host or device allocation failed            */
+  SBLAS_ERR_UNSUPPORTED = 7,   /* entry point without a defined reference     */
+  SBLAS_ERR_NO_DEVICE = 8      /* no HIP device visible                        */
+};
+int sblas_last_error(void);
+const char *sblas_error_string(int status);
+const char *sblas_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* 1. drop-in entry points                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* include/spmv_kernel.h:16-57: five timings in seconds, same field order. */
+typedef struct spmv_ret {
+  double numa_part_time;
+  double part_time;
+  double comp_time;
+  double comm_time;
+  double merg_time;
+} spmv_ret;
+
+/* src/dspmv_mgpu_baseline.cu:14 -- row blocks */
+spmv_ret spMV_mgpu_baseline(int m, int n, int nnz, double *alpha, double *csrVal,
+                            int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                            double *y, int ngpu);
+/* src/dspmv_mgpu_baseline_csc.cu:14 -- column blocks */
+spmv_ret spMV_mgpu_baseline_csc(int m, int n, int nnz, double *alpha, double *cscVal,
+                                int *cscColPtr, int *cscRowIdx, double *x, double *beta,
+                                double *y, int ngpu);
+/* src/dspmv_mgpu_baseline_coo.cu:16 -- row blocks of row-sorted COO */
+spmv_ret spMV_mgpu_baseline_coo(int m, int n, int nnz, double *alpha, double *cooVal,
+                                int *cooRowIdx, int *cooColIdx, double *x, double *beta,
+                                double *y, int ngpu);
+/* src/dspmv_mgpu_v1.cu:76 -- nnz chunks, kernel 1 (vector CSR), 2 (merge
+ * path), 3 (CSR5-style tiles) */
+spmv_ret spMV_mgpu_v1(int m, int n, int nnz, double *alpha, double *csrVal,
+                      int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                      double *y, int ngpu, int kernel);
+/* src/dspmv_mgpu_v1_numa.cu:98 -- two-level nnz chunks; kernel honoured */
+spmv_ret spMV_mgpu_v1_numa(int m, int n, int nnz, double *alpha, double *csrVal,
+                           int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                           double *y, int ngpu, int kernel, int *numa_mapping,
+                           int part_opt, int merg_opt);
+/* src/dspmv_mgpu_v1_numa_csc.cu:43 */
+spmv_ret spMV_mgpu_v1_numa_csc(int m, int n, long long nnz, double *alpha, double *cscVal,
+                               int *cscColPtr, int *cscRowIndex, double *x, double *beta,
+                               double *y, int ngpu, int kernel, int *numa_mapping,
+                               int part_opt, int merg_opt);
+/* src/dspmv_mgpu_v1_numa_coo.cu:35 */
+spmv_ret spMV_mgpu_v1_numa_coo(int m, int n, int nnz, double *alpha, double *cooVal,
+                               int *cooRowIdx, int *cooColIdx, double *x, double *beta,
+                               double *y, int ngpu, int kernel, int *numa_mapping,
+                               int part_opt, int merg_opt);
+/* src/dspmv_mgpu_v2.cu:103 -- nb-sized tasks over copy_of_workspace streams */
+spmv_ret spMV_mgpu_v2(int m, int n, int nnz, double *alpha, double *csrVal,
+                      int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                      double *y, int ngpu, int kernel, int nb, int copy_of_workspace);
+/* src/dspmv_mgpu_v1.cu:19 -- drops entries whose x[col] == 0, then v1 */
+spmv_ret spMspV_mgpu_v1(int m, int n, int nnz, double *alpha, double *csrVal,
+                        int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                        double *y, int ngpu, int kernel);
+/* include/spmv_kernel.h:282 -- declared but never defined by the reference;
+ * returns NaN timings with SBLAS_ERR_UNSUPPORTED and leaves y untouched. */
+spmv_ret spMspV_mgpu_v1_numa(int m, int n, int nnz, double *alpha, double *csrVal,
+                             int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                             double *y, int ngpu, int kernel);
+/* src/dspmv_mgpu_v2.cu:35 */
+spmv_ret spMspV_mgpu_v2(int m, int n, int nnz, double *alpha, double *csrVal,
+                        int *csrRowPtr, int *csrColIndex, double *x, double *beta,
+                        double *y, int ngpu, int kernel, int nb, int copy_of_workspace);
+
+/* src/spmv_helper.cu:18-44, :646-685 -- the reference's searches */
+int get_row_from_index(int n, int *a, int idx);
+int findFirstInSorted(int *a, int n, int key);
+int findLastInSorted(int *a, int n, int key);
+
+/* ---------------------------------------------------------------------- */
+/* 2. partition API                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* One chunk (pCSR / pCOO / pCSC / spmv_task fields, include/spmv_kernel.h:
+ * 59-142, include/spmv_task.h:4-35) with indices made global.  For CSC
+ * chunks start_row/end_row are columns and m_c is the column count. */
+typedef struct sblas_chunk_desc {
+  int64_t start_idx;
+  int64_t end_idx;
+  int32_t start_row;
+  int32_t end_row;
+  int32_t start_flag;
+  int32_t end_flag;
+  int32_t m_c;
+  int32_t pad_;
+  int64_t nnz_c;
+} sblas_chunk_desc;
+
+/* Host arrays.  Return SBLAS_OK, SBLAS_ERR_ARGS, or SBLAS_ERR_UNDEFINED where
+ * the reference's own arithmetic is undefined (then out[] is unspecified). */
+/* src/dspmv_mgpu_baseline.cu:64-70 */
+int sblas_partition_rowblock(int m, const int *row_ptr, int C, sblas_chunk_desc *out);
+/* src/dspmv_mgpu_baseline_coo.cu:65-73 */
+int sblas_partition_baseline_coo(int m, int nnz, const int *row_idx, int C,
+                                 sblas_chunk_desc *out);
+/* src/dspmv_mgpu_baseline_csc.cu:61-67 */
+int sblas_partition_baseline_csc(int n, const int *col_ptr, int C, sblas_chunk_desc *out);
+/* src/dspmv_mgpu_v1.cu:119-157 */
+int sblas_partition_v1(int m, const int *row_ptr, int nnz, int C, sblas_chunk_desc *out);
+/* src/dspmv_mgpu_v1_numa.cu:210-255, :432-497.  numa_mapping: C entries or
+ * NULL (one node).  node_out: optional, max(mapping)+1 entries. */
+int sblas_partition_v1_numa(int m, const int *row_ptr, int nnz, int C,
+                            const int *numa_mapping, sblas_chunk_desc *out,
+                            sblas_chunk_desc *node_out);
+/* src/dspmv_mgpu_v1_numa_coo.cu:75-111, :211-250 */
+int sblas_partition_v1_numa_coo(int m, int nnz, const int *row_idx, int C,
+                                const int *numa_mapping, sblas_chunk_desc *out,
+                                sblas_chunk_desc *node_out);
+/* src/dspmv_mgpu_v1_numa_csc.cu:79-108, :220-250 */
+int sblas_partition_v1_numa_csc(int n, const int *col_ptr, long long nnz, int C,
+                                const int *numa_mapping, sblas_chunk_desc *out,
+                                sblas_chunk_desc *node_out);
+/* src/dspmv_mgpu_v2.cu:114, :318-366.  Returns the task count T > 0 or
+ * -status; out must hold T entries (T from sblas_v2_num_tasks). */
+int sblas_v2_num_tasks(int nnz, int nb, int q);
+int sblas_partition_v2(int m, const int *row_ptr, int nnz, int nb, int q,
+                       sblas_chunk_desc *out, int out_cap);
+
+/* ---------------------------------------------------------------------- */
+/* 3. plan API (device pointers)                                           */
+/* ---------------------------------------------------------------------- */
+
+enum { SBLAS_VAL_F64 = 0, SBLAS_VAL_F32 = 1, SBLAS_VAL_BF16 = 2 };
+enum { SBLAS_FMT_CSR = 0, SBLAS_FMT_COO = 1, SBLAS_FMT_CSC = 2 };
+enum {
+  SBLAS_KERNEL_AUTO = 0,
+  SBLAS_KERNEL_VECTOR = 1,  /* vector CSR, wave64 split into sub-waves      */
+  SBLAS_KERNEL_MERGE = 2,   /* merge-path CSR                               */
+  SBLAS_KERNEL_CSR5 = 3     /* CSR5-style tiles, wave64 segmented sum       */
+};
+enum {
+  SBLAS_PART_V1 = 0,        /* src/dspmv_mgpu_v1.cu:119-130                 */
+  SBLAS_PART_V1_NUMA = 1,   /* src/dspmv_mgpu_v1_numa.cu:210-214,:432-436    */
+  SBLAS_PART_V2 = 2,        /* src/dspmv_mgpu_v2.cu:318-338 (nb per task)    */
+  SBLAS_PART_ROWBLOCK = 3   /* src/dspmv_mgpu_baseline.cu:64-70             */
+};
+
+typedef struct sblas_opts {
+  int kernel;      /* SBLAS_KERNEL_*; AUTO lets the autotuner choose        */
+  int chunks;      /* chunk count C (0 = autotuner)                         */
+  int part_mode;   /* SBLAS_PART_*                                          */
+  int nb;          /* v2 task size in nnz (part_mode V2)                    */
+  int streams;     /* >1: one launch per chunk over this many streams       */
+  int subwave;     /* vector kernel lanes per row (0 = from row lengths)    */
+  int sigma;       /* CSR5 nnz per lane per tile (0 = autotuned)            */
+  int xcd_remap;   /* 1: contiguous row ranges per XCD; 0: off; -1: auto   */
+  int reserved[8];
+} sblas_opts;
+
+/* Fills opts with defaults (auto kernel, C=1, v1 partition, fused launch). */
+void sblas_opts_default(sblas_opts *opts);
+
+typedef struct sblas_plan_s *sblas_plan;
+
+/* CSR: d_row_ptr[m+1], d_col[nnz], d_val[nnz] of val_type, device memory the
+ * caller keeps alive and unchanged for the plan's lifetime. */
+int sblas_plan_create_csr(int m, int n, int64_t nnz, const int *d_row_ptr,
+                          const int *d_col, const void *d_val, int val_type,
+                          const sblas_opts *opts, sblas_plan *plan);
+/* COO sorted by row: d_row_idx[nnz], d_col_idx[nnz], d_val[nnz]. */
+int sblas_plan_create_coo(int m, int n, int64_t nnz, const int *d_row_idx,
+                          const int *d_col_idx, const void *d_val, int val_type,
+                          const sblas_opts *opts, sblas_plan *plan);
+/* CSC: d_col_ptr[n+1], d_row_idx[nnz], d_val[nnz]. */
+int sblas_plan_create_csc(int m, int n, int64_t nnz, const int *d_col_ptr,
+                          const int *d_row_idx, const void *d_val, int val_type,
+                          const sblas_opts *opts, sblas_plan *plan);
+/* y = alpha*A*x + beta*y on `stream` (a hipStream_t; NULL = default).
+ * beta == 0 does not read y. Asynchronous. */
+int sblas_plan_spmv(sblas_plan plan, double alpha, const double *d_x, double beta,
+                    double *d_y, void *stream);
+/* Y = alpha*A*X + beta*Y, X row-major n x k, Y row-major m x k (FP64). */
+int sblas_plan_spmm(sblas_plan plan, int k, double alpha, const double *d_X,
+                    double beta, double *d_Y, void *stream);
+typedef struct sblas_plan_info {
+  int format, val_type, kernel, chunks, subwave, sigma, xcd_remap, split_rows;
+  int64_t m, n, nnz;
+  int64_t workspace_bytes;
+  double mean_row, cv_row;
+  int64_t max_row;
+  int launches;      /* kernel launches per sblas_plan_spmv                 */
+  int reserved[7];
+} sblas_plan_info;
+int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info);
+/* Chunk descriptors of the plan (reference rule of opts.part_mode). */
+int sblas_plan_get_chunks(sblas_plan plan, sblas_chunk_desc *out, int cap);
+int sblas_plan_destroy(sblas_plan plan);
+/* Measurement: with timing on, every sblas_plan_spmv / _spmm records HIP
+ * events around its main kernel launch (fused mode) on the call's stream;
+ * sblas_plan_kernel_times waits for them, writes up to `cap` durations in
+ * milliseconds, clears the record and returns how many it wrote. */
+int sblas_plan_set_timing(sblas_plan plan, int on);
+int sblas_plan_kernel_times(sblas_plan plan, float *ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBLAS_H */

@@ -1,0 +1,88 @@
+"""The C ABI: the library loads, exports every function include/sblas.h
+declares, and rejects bad calls with a status (never exiting), without a GPU."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "sblas.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text))
+    return sorted(n for n in names if n not in {"if", "sizeof", "defined"} and
+                  not n.isupper())
+
+
+def test_every_declared_symbol_is_exported():
+    from hpda_sblas_pnnl_amd import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert "spMV_mgpu_v1" in names and "sblas_plan_spmv" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def _csr_small():
+    rp = np.array([0, 2, 3], np.int32)
+    col = np.array([0, 1, 1], np.int32)
+    val = np.array([1.0, 2.0, 3.0])
+    x = np.array([1.0, 1.0])
+    return rp, col, val, x
+
+
+def _nan(ret):
+    return all(math.isnan(getattr(ret, f)) for f, _ in ret._fields_)
+
+
+def test_error_statuses_without_gpu():
+    from hpda_sblas_pnnl_amd import _lib
+    L = _lib.lib()
+    rp, col, val, x = _csr_small()
+    y = np.zeros(2)
+    a, b = ctypes.c_double(1.0), ctypes.c_double(0.0)
+    args = (2, 2, 3, ctypes.byref(a), val.ctypes.data, rp.ctypes.data, col.ctypes.data,
+            x.ctypes.data, ctypes.byref(b), y.ctypes.data)
+    assert _nan(L.spMV_mgpu_v1(*args, 2, 1)) and L.sblas_last_error() == _lib.SBLAS_ERR_NGPU
+    assert _nan(L.spMV_mgpu_v1(*args, 1, 4)) and L.sblas_last_error() == _lib.SBLAS_ERR_KERNEL
+    assert _nan(L.spMV_mgpu_v2(*args, 1, 1, 0, 1)) and L.sblas_last_error() == _lib.SBLAS_ERR_ARGS
+    assert _nan(L.spMspV_mgpu_v1_numa(*args, 1, 1))
+    assert L.sblas_last_error() == _lib.SBLAS_ERR_UNSUPPORTED
+    bad = (-1,) + args[1:]
+    assert _nan(L.spMV_mgpu_baseline(*bad, 1)) and L.sblas_last_error() == _lib.SBLAS_ERR_ARGS
+    rp_bad = np.array([0, 2, 4], np.int32)  # rp[m] != nnz
+    args2 = args[:5] + (rp_bad.ctypes.data,) + args[6:]
+    assert _nan(L.spMV_mgpu_v1(*args2, 1, 1)) and L.sblas_last_error() == _lib.SBLAS_ERR_ARGS
+    assert np.array_equal(y, np.zeros(2))
+    assert L.sblas_error_string(_lib.SBLAS_ERR_NGPU) == b"ngpu must be 1"
+
+
+def test_python_mirror_raises():
+    from hpda_sblas_pnnl_amd import spmv
+    from hpda_sblas_pnnl_amd._lib import SblasError
+    rp, col, val, x = _csr_small()
+    y = np.zeros(2)
+    with pytest.raises(SblasError):
+        spmv.spMV_mgpu_v1(2, 2, 3, 1.0, val, rp, col, x, 0.0, y, ngpu=2, kernel=1)
+    with pytest.raises(TypeError):
+        spmv.spMV_mgpu_v1(2, 2, 3, 1.0, val, rp, col, x, 0.0, [0.0, 0.0])
+
+
+def test_no_device_fails_loudly():
+    """Without a HIP device a valid call reports an error instead of
+    computing anything on the host."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from hpda_sblas_pnnl_amd import _lib, spmv
+    rp, col, val, x = _csr_small()
+    y = np.zeros(2)
+    with pytest.raises(_lib.SblasError) as e:
+        spmv.spMV_mgpu_v1(2, 2, 3, 1.0, val, rp, col, x, 0.0, y)
+    assert e.value.status in (_lib.SBLAS_ERR_NO_DEVICE, _lib.SBLAS_ERR_HIP)
