@@ -121,6 +121,10 @@ def main():
     ap.add_argument("--workload", default="banded", choices=["banded", "rmat", "powerlaw"])
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--chunks", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="kernel 1 form: 1 sub-wave, 2 row blocks")
+    ap.add_argument("--items", type=int, default=0)
+    ap.add_argument("--subwave", type=int, default=0)
+    ap.add_argument("--xcd-remap", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
 
@@ -144,7 +148,9 @@ def main():
     dy0 = torch.from_numpy(y0).to(dev)
     dy = dy0.clone()
     kernel = int(args.kernel) if args.kernel.isdigit() else args.kernel
-    plan = SpmvPlan("csr", A.m, A.n, rp, col, val, kernel=kernel, chunks=chunks, part_mode=part)
+    plan = SpmvPlan("csr", A.m, A.n, rp, col, val, kernel=kernel, chunks=chunks, part_mode=part,
+                    variant=args.variant, items=args.items, subwave=args.subwave,
+                    xcd_remap=args.xcd_remap)
     info = plan.info()
     log(f"[bench] plan: {info}")
 
@@ -186,7 +192,9 @@ def main():
         "data": "synthetic (seeded generator)",
         "config": {"workload": desc, "m": A.m, "n": A.n, "nnz": A.nnz, "alpha": alpha,
                    "beta": beta, "chunks": info["chunks"], "kernel": info["kernel"],
-                   "subwave": info["subwave"], "parallelism": "single GPU"},
+                   "subwave": info["subwave"], "variant": info["variant"],
+                   "items": info["items"], "xcd_remap": info["xcd_remap"],
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": load_traffic(args.workload),

@@ -25,6 +25,8 @@ ARCH = os.environ.get("SBLAS_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = [
     "sblas_vector.hip",
     "sblas_tiles.hip",
+    "sblas_csc.hip",
+    "sblas_csr5.hip",
     "plan.cpp",
     "dropin.cpp",
     "partition.cpp",

@@ -84,12 +84,13 @@ struct EventPair {
   }
 };
 
-// CSR core of every entry point: host arrays in, y out.
-spmv_ret run_csr(int m, int n, int nnz, const double *alpha, const double *val, const int *rp,
-                 const int *col, const double *x, const double *beta, double *y, int kernel,
-                 int part_mode, int nb, int q, double host_prep_s) {
+// Core of every entry point: host arrays in, y out.  format CSR: a0 = row
+// pointers (m+1), a1 = columns; COO: a0 = row indices (nnz), a1 = columns;
+// CSC: a0 = column pointers (n+1), a1 = row indices.
+spmv_ret run_format(int format, int m, int n, int nnz, const double *alpha, const double *val,
+                    const int *a0, const int *a1, const double *x, const double *beta, double *y,
+                    int kernel, int part_mode, int nb, int q, double host_prep_s) {
   spmv_ret ret{0, 0, 0, 0, 0};
-  const double t_part0 = now_s();
   DeviceGuard guard(env_int("SBLAS_DEVICE", 0));
   if (!guard.ok) {
     set_error(SBLAS_ERR_NO_DEVICE, "no HIP device");
@@ -105,21 +106,24 @@ spmv_ret run_csr(int m, int n, int nnz, const double *alpha, const double *val, 
     ~StreamGuard() { (void)hipStreamDestroy(s); }
   } sg{s};
   DeviceArrays mem;
-  int *d_rp = mem.alloc<int>(static_cast<size_t>(m) + 1);
+  const size_t a0_len = format == SBLAS_FMT_CSR   ? static_cast<size_t>(m) + 1
+                        : format == SBLAS_FMT_CSC ? static_cast<size_t>(n) + 1
+                                                  : static_cast<size_t>(nnz);
+  int *d_a0 = mem.alloc<int>(a0_len);
   int *d_col = mem.alloc<int>(nnz);
   double *d_val = mem.alloc<double>(nnz);
   double *d_x = mem.alloc<double>(n);
   double *d_y = mem.alloc<double>(m);
-  if (!d_rp || !d_col || !d_val || !d_x || !d_y) {
+  if (!d_a0 || !d_col || !d_val || !d_x || !d_y) {
     set_error(SBLAS_ERR_ALLOC, "device allocation failed");
     return nan_ret();
   }
   EventPair up, comp, merge, down;
   (void)hipEventRecord(up.a, s);
-  bool ok = hipMemcpyAsync(d_rp, rp, sizeof(int) * (static_cast<size_t>(m) + 1),
-                           hipMemcpyHostToDevice, s) == hipSuccess;
+  bool ok = a0_len == 0 ||
+            hipMemcpyAsync(d_a0, a0, sizeof(int) * a0_len, hipMemcpyHostToDevice, s) == hipSuccess;
   if (nnz > 0) {
-    ok = ok && hipMemcpyAsync(d_col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(d_col, a1, sizeof(int) * nnz, hipMemcpyHostToDevice, s) == hipSuccess;
     ok = ok && hipMemcpyAsync(d_val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s) == hipSuccess;
   }
   if (n > 0) ok = ok && hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, s) == hipSuccess;
@@ -136,28 +140,29 @@ spmv_ret run_csr(int m, int n, int nnz, const double *alpha, const double *val, 
   o.nb = nb;
   o.streams = q;
   o.chunks = std::max(1, env_int("SBLAS_CHUNKS", 1));
-  if (part_mode == SBLAS_PART_ROWBLOCK) o.chunks = std::max(1, env_int("SBLAS_CHUNKS", 1));
   sblas_plan plan = nullptr;
   const double t_plan0 = now_s();
-  int st = sblas_plan_create_csr(m, n, nnz, d_rp, d_col, d_val, SBLAS_VAL_F64, &o, &plan);
+  int st = format == SBLAS_FMT_CSR
+               ? sblas_plan_create_csr(m, n, nnz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan)
+           : format == SBLAS_FMT_COO
+               ? sblas_plan_create_coo(m, n, nnz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan)
+               : sblas_plan_create_csc(m, n, nnz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan);
   if (st != SBLAS_OK) return nan_ret();
   struct PlanGuard {
     sblas_plan p;
     ~PlanGuard() { sblas_plan_destroy(p); }
   } pg{plan};
-  ret.part_time = host_prep_s + (now_s() - t_plan0) + (t_plan0 - t_part0) * 0.0;
+  ret.part_time = host_prep_s + (now_s() - t_plan0);
   // kernels, then the split-row merge, each between its own events
   Plan &p = plan->impl;
-  const int groups = p.ngroups;
-  p.ngroups = 0;
+  const bool merged = p.has_merge(1);
+  p.merge_enabled = false;
   (void)hipEventRecord(comp.a, s);
   st = p.run_spmv(*alpha, d_x, *beta, d_y, s);
   (void)hipEventRecord(comp.b, s);
-  p.ngroups = groups;
-  if (st == SBLAS_OK && groups > 0) {
+  if (st == SBLAS_OK && merged) {
     (void)hipEventRecord(merge.a, s);
-    st = launch_merge(p.d_mrows, p.d_mptr, p.d_mslots, groups, p.d_partials, 1, d_y, *alpha,
-                      *beta, s);
+    st = p.run_merge(1, *alpha, *beta, d_y, s);
     (void)hipEventRecord(merge.b, s);
   }
   if (st != SBLAS_OK) return nan_ret();
@@ -173,9 +178,16 @@ spmv_ret run_csr(int m, int n, int nnz, const double *alpha, const double *val, 
   }
   ret.comm_time = up.seconds() + down.seconds();
   ret.comp_time = comp.seconds();
-  ret.merg_time = groups > 0 ? merge.seconds() : 0.0;
+  ret.merg_time = merged ? merge.seconds() : 0.0;
   ret.numa_part_time = 0.0;
   return ret;
+}
+
+spmv_ret run_csr(int m, int n, int nnz, const double *alpha, const double *val, const int *rp,
+                 const int *col, const double *x, const double *beta, double *y, int kernel,
+                 int part_mode, int nb, int q, double host_prep_s) {
+  return run_format(SBLAS_FMT_CSR, m, n, nnz, alpha, val, rp, col, x, beta, y, kernel, part_mode,
+                    nb, q, host_prep_s);
 }
 
 bool check_common(int m, int n, int64_t nnz, const double *alpha, const double *beta,
@@ -207,49 +219,6 @@ bool check_csr(int m, int nnz, const int *rp) {
     set_error(SBLAS_ERR_ARGS, "csrRowPtr must start at 0 and end at nnz");
     return false;
   }
-  return true;
-}
-
-// Row-sorted COO -> row pointers (the rows the reference's coo2csr would give).
-bool coo_row_ptr(int m, int nnz, const int *ri, std::vector<int> &rp) {
-  rp.assign(static_cast<size_t>(m) + 1, 0);
-  for (int i = 0; i < nnz; i++) {
-    if (ri[i] < 0 || ri[i] >= m || (i > 0 && ri[i] < ri[i - 1])) {
-      set_error(SBLAS_ERR_ARGS, "COO row indices must be sorted and in [0, m)");
-      return false;
-    }
-    rp[ri[i] + 1]++;
-  }
-  for (int r = 0; r < m; r++) rp[r + 1] += rp[r];
-  return true;
-}
-
-// CSC -> CSR by a stable counting transpose (row-major order, columns
-// ascending within a row).
-bool csc_to_csr(int m, int n, int nnz, const double *cv, const int *cp, const int *ri,
-                std::vector<int> &rp, std::vector<int> &col, std::vector<double> &val) {
-  if (cp[0] != 0 || cp[n] != nnz) {
-    set_error(SBLAS_ERR_ARGS, "cscColPtr must start at 0 and end at nnz");
-    return false;
-  }
-  rp.assign(static_cast<size_t>(m) + 1, 0);
-  for (int i = 0; i < nnz; i++) {
-    if (ri[i] < 0 || ri[i] >= m) {
-      set_error(SBLAS_ERR_ARGS, "CSC row index out of range");
-      return false;
-    }
-    rp[ri[i] + 1]++;
-  }
-  for (int r = 0; r < m; r++) rp[r + 1] += rp[r];
-  col.resize(nnz);
-  val.resize(nnz);
-  std::vector<int> next(rp.begin(), rp.end() - 1);
-  for (int c = 0; c < n; c++)
-    for (int j = cp[c]; j < cp[c + 1]; j++) {
-      const int pos = next[ri[j]]++;
-      col[pos] = c;
-      val[pos] = cv[j];
-    }
   return true;
 }
 
@@ -385,11 +354,9 @@ spmv_ret spMV_mgpu_baseline_coo(int m, int n, int nnz, double *alpha, double *co
                                 double *y, int ngpu) {
   if (!check_common(m, n, nnz, alpha, beta, cooVal, cooRowIdx, cooColIdx, x, y, ngpu))
     return nan_ret();
-  const double t0 = now_s();
-  std::vector<int> rp;
-  if (!coo_row_ptr(m, nnz, cooRowIdx, rp)) return nan_ret();
-  return run_csr(m, n, nnz, alpha, cooVal, rp.data(), cooColIdx, x, beta, y, 1,
-                 SBLAS_PART_ROWBLOCK, 0, 1, now_s() - t0);
+  // COO consumed directly by the segmented-reduction kernel (no conversion)
+  return run_format(SBLAS_FMT_COO, m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y,
+                    SBLAS_KERNEL_COO, SBLAS_PART_ROWBLOCK, 0, 1, 0.0);
 }
 
 spmv_ret spMV_mgpu_v1_numa_coo(int m, int n, int nnz, double *alpha, double *cooVal,
@@ -400,11 +367,8 @@ spmv_ret spMV_mgpu_v1_numa_coo(int m, int n, int nnz, double *alpha, double *coo
   if (!check_common(m, n, nnz, alpha, beta, cooVal, cooRowIdx, cooColIdx, x, y, ngpu) ||
       !check_kernel(kernel))
     return nan_ret();
-  const double t0 = now_s();
-  std::vector<int> rp;
-  if (!coo_row_ptr(m, nnz, cooRowIdx, rp)) return nan_ret();
-  return run_csr(m, n, nnz, alpha, cooVal, rp.data(), cooColIdx, x, beta, y, kernel,
-                 SBLAS_PART_V1_NUMA, 0, 1, now_s() - t0);
+  return run_format(SBLAS_FMT_COO, m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y,
+                    SBLAS_KERNEL_COO, SBLAS_PART_V1_NUMA, 0, 1, 0.0);
 }
 
 spmv_ret spMV_mgpu_baseline_csc(int m, int n, int nnz, double *alpha, double *cscVal,
@@ -412,12 +376,9 @@ spmv_ret spMV_mgpu_baseline_csc(int m, int n, int nnz, double *alpha, double *cs
                                 double *y, int ngpu) {
   if (!check_common(m, n, nnz, alpha, beta, cscVal, cscColPtr, cscRowIdx, x, y, ngpu))
     return nan_ret();
-  const double t0 = now_s();
-  std::vector<int> rp, col;
-  std::vector<double> val;
-  if (!csc_to_csr(m, n, nnz, cscVal, cscColPtr, cscRowIdx, rp, col, val)) return nan_ret();
-  return run_csr(m, n, nnz, alpha, val.data(), rp.data(), col.data(), x, beta, y, 1,
-                 SBLAS_PART_V1, 0, 1, now_s() - t0);
+  // stable device transpose to row-major form, then the CSR kernels
+  return run_format(SBLAS_FMT_CSC, m, n, nnz, alpha, cscVal, cscColPtr, cscRowIdx, x, beta, y,
+                    SBLAS_KERNEL_AUTO, SBLAS_PART_ROWBLOCK, 0, 1, 0.0);
 }
 
 spmv_ret spMV_mgpu_v1_numa_csc(int m, int n, long long nnz, double *alpha, double *cscVal,
@@ -428,13 +389,8 @@ spmv_ret spMV_mgpu_v1_numa_csc(int m, int n, long long nnz, double *alpha, doubl
   if (!check_common(m, n, nnz, alpha, beta, cscVal, cscColPtr, cscRowIndex, x, y, ngpu) ||
       !check_kernel(kernel))
     return nan_ret();
-  const double t0 = now_s();
-  std::vector<int> rp, col;
-  std::vector<double> val;
-  if (!csc_to_csr(m, n, static_cast<int>(nnz), cscVal, cscColPtr, cscRowIndex, rp, col, val))
-    return nan_ret();
-  return run_csr(m, n, static_cast<int>(nnz), alpha, val.data(), rp.data(), col.data(), x, beta,
-                 y, kernel, SBLAS_PART_V1, 0, 1, now_s() - t0);
+  return run_format(SBLAS_FMT_CSC, m, n, static_cast<int>(nnz), alpha, cscVal, cscColPtr,
+                    cscRowIndex, x, beta, y, kernel, SBLAS_PART_V1_NUMA, 0, 1, 0.0);
 }
 
 // ---- searches and partitions ---------------------------------------------------

@@ -49,6 +49,30 @@ struct TileCarry {
   int pad_;
 };
 
+// Row block of the LDS-staged CSR kernel: rows [row_begin, row_end) and the
+// element window [lo, hi) they own (clamped at chunk edges and cut into
+// pieces for rows longer than one block).  Split rows write partials to slot
+// 2*block (first row) or 2*block+1 (last row).
+struct RowBlock {
+  int row_begin, row_end;
+  int lo, hi;
+};
+
+constexpr int kStreamBlock = 256;      // threads per row-block launch block
+constexpr int kStreamRowsMax = 1024;   // rows per row block (LDS y staging)
+
+struct StreamLaunch {
+  int items;        // elements per thread: block holds 256*items elements
+  int subwave;      // lanes per row in the LDS reduction (1..64)
+  int nblocks;
+  int grid;
+  int xcd_remap;
+};
+
+int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *blocks,
+                      const int *rp, const int *col, const void *val, const double *x,
+                      double *y, double alpha, double beta, double *partials, hipStream_t s);
+
 // ---- launchers (sblas_kernels.hip) --------------------------------------------
 struct VectorLaunch {
   int subwave;          // lanes per row: 2..64
@@ -70,29 +94,35 @@ int launch_merge(const int *rows, const int *ptr, const int *slots, int groups,
                  const double *partials, int k, double *y, double alpha, double beta,
                  hipStream_t s);
 int launch_scale(int64_t count, double beta, double *y, hipStream_t s);
+// *ok = every a[i] in [lo, hi) (and non-decreasing when sorted); synchronous.
+int check_indices(const int *a, int64_t count, int lo, int hi, bool sorted, hipStream_t s,
+                  bool *ok);
+// CSC -> CSR on the device, stable by row (sblas_csc.hip).
+int csc_to_csr_device(int m, int n, int nnz, const int *cp, const int *ri, const void *val,
+                      int val_type, int *rp, int *col, void *val_out, hipStream_t s);
 
-// segmented tiles
-int launch_coo_tiles(int val_type, int64_t nnz, int tiles, const int *row_idx,
+// segmented tiles (sblas_tiles.hip): merge-path CSR and COO
+constexpr int kTileBlock = 256;   // threads per tile block
+constexpr int kTileItems = 8;     // merge items (or COO elements) per thread per tile
+constexpr int kTileSize = kTileBlock * kTileItems;
+
+int tile_subwave(double mean_row);  // 1, 4, 16 or 64 lanes per row
+int launch_csr_merge_path(int val_type, int sub, int m, int64_t nnz, int tiles, const int *rp,
+                          const int *col, const void *val, const double *x, double *y,
+                          double alpha, double beta, TileCarry *carry, hipStream_t s);
+int launch_coo_tiles(int val_type, int sub, int m, int64_t nnz, int tiles, const int *row_idx,
                      const int *col_idx, const void *val, const double *x, double *y,
                      double alpha, double beta, TileCarry *carry, hipStream_t s);
 int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha, double beta,
                       hipStream_t s);
-int launch_csr_merge_path(int val_type, int m, int64_t nnz, int tiles, const int *rp,
-                          const int *col, const void *val, const double *x, double *y,
-                          double alpha, double beta, TileCarry *carry, hipStream_t s);
-int launch_csr5(int val_type, int m, int64_t nnz, int tiles, int sigma, const int *tile_row,
-                const uint32_t *tile_flags, const int *rp, const int *col, const void *val,
+
+// CSR5-style tiles (sblas_csr5.hip): 64 lanes x sigma elements per tile
+int launch_csr5(int val_type, int sigma, int ntiles, int64_t nnz, const uint32_t *tile_nz,
+                const uint32_t *flags, const int *rows_nz, const int *colT, const void *valT,
                 const double *x, double *y, double alpha, double beta, TileCarry *carry,
                 hipStream_t s);
-int launch_csr5_build(int m, int64_t nnz, int tiles, int sigma, const int *rp, int *tile_row,
-                      uint32_t *tile_flags, hipStream_t s);
-int launch_empty_rows(int m, const int *rp, double *y, double beta, hipStream_t s);
-
-// CSC
-int launch_csc_count(int m, int n, int64_t nnz, const int *cp, const int *ri, int *row_cnt,
-                     hipStream_t s);
-
-constexpr int kTileBlock = 256;   // threads per segmented-tile block
-constexpr int kTileItems = 8;     // elements per thread per tile (COO / merge path)
+int launch_csr5_transpose(int val_type, int sigma, int64_t padded, int64_t nnz, const int *col,
+                          const void *val, int *colT, void *valT, hipStream_t s);
+int launch_scale_rows(int count, const int *rows, double beta, double *y, hipStream_t s);
 
 }  // namespace sblas

@@ -71,14 +71,14 @@ RowStats row_stats(int m, const int *rp) {
   return s;
 }
 
-// Kernel choice from row-length statistics: rows of similar length go to the
-// vector kernel (one sub-wave per row); heavy-tailed rows, where one row can
-// be thousands of times the mean, go to the merge path, whose work per block
-// is bounded whatever the row lengths.
+// Kernel choice from row-length statistics.  The row-block form of kernel 1
+// packs short rows and cuts long rows into block-sized pieces, so it stays
+// balanced on heavy-tailed matrices too; measured on MI355X it beats the
+// merge path on R-MAT (0.51 vs 0.76 ms at scale 22) and on the banded
+// matrix, so it is the default for every row-length profile.
 int autotune_kernel(const RowStats &s) {
-  const bool heavy =
-      (s.cv > 1.5) || (s.max_row > 64 * std::max(1.0, s.mean) && s.max_row > 4096);
-  return heavy ? SBLAS_KERNEL_MERGE : SBLAS_KERNEL_VECTOR;
+  (void)s;
+  return SBLAS_KERNEL_VECTOR;
 }
 
 // Lanes per row: about three quarters of the mean row length, so most rows
@@ -245,7 +245,10 @@ int Plan::build_csr_chunks(const int *h_rp) {
 
 int Plan::setup_vector(const int *h_rp) {
   (void)h_rp;
-  int sub = opts.subwave > 0 ? opts.subwave : autotune_subwave(stats);
+  // the sub-wave option targets the kernel in use; this form needs 2..64
+  int sub = (kernel == SBLAS_KERNEL_VECTOR && variant == 1 && opts.subwave >= 2)
+                ? opts.subwave
+                : autotune_subwave(stats);
   if (sub < 2 || sub > 64 || (sub & (sub - 1))) {
     set_error(SBLAS_ERR_ARGS, "subwave must be a power of two in [2, 64]");
     return SBLAS_ERR_ARGS;
@@ -280,6 +283,77 @@ int Plan::setup_vector(const int *h_rp) {
   return SBLAS_OK;
 }
 
+// Row blocks over the execution chunks: consecutive rows are packed while
+// their elements fit one block (cap elements, at most kStreamRowsMax rows);
+// a row with more elements than that inside its chunk is cut into cap-sized
+// single-row pieces.  Every row of [0, m) lands in some block.
+std::vector<RowBlock> build_row_blocks(const std::vector<ExecChunk> &ex, const int *rp,
+                                       int cap) {
+  std::vector<RowBlock> out;
+  for (const ExecChunk &ch : ex) {
+    const int64_t lo = ch.lo, hi = ch.hi;
+    int r = ch.row_begin;
+    while (r < ch.row_end) {
+      const int64_t s = std::max<int64_t>(rp[r], lo);
+      const int64_t e = std::min<int64_t>(rp[r + 1], hi);
+      if (e - s > cap) {
+        for (int64_t p = s; p < e; p += cap)
+          out.push_back(RowBlock{r, r + 1, static_cast<int>(p),
+                                 static_cast<int>(std::min<int64_t>(p + cap, e))});
+        r++;
+        continue;
+      }
+      const int rb = r;
+      r++;
+      while (r < ch.row_end && r - rb < kStreamRowsMax &&
+             std::min<int64_t>(rp[r + 1], hi) - s <= cap)
+        r++;
+      const int64_t bhi = std::min<int64_t>(std::max<int64_t>(rp[r], lo), hi);
+      out.push_back(RowBlock{rb, r, static_cast<int>(s), static_cast<int>(std::max(bhi, s))});
+    }
+  }
+  return out;
+}
+
+int Plan::setup_stream(const int *h_rp) {
+  int items = opts.items > 0 ? opts.items : 8;
+  if (items != 8 && items != 16) {
+    set_error(SBLAS_ERR_ARGS, "items must be 8 or 16");
+    return SBLAS_ERR_ARGS;
+  }
+  const int cap = kStreamBlock * items;
+  rblocks = build_row_blocks(exec, h_rp, cap);
+  int sub = opts.subwave > 0 ? opts.subwave : pow2_round(stats.mean / 4.0, 1, 64);
+  if (sub < 1 || sub > 64 || (sub & (sub - 1))) {
+    set_error(SBLAS_ERR_ARGS, "subwave must be a power of two in [1, 64]");
+    return SBLAS_ERR_ARGS;
+  }
+  int remap = opts.xcd_remap;
+  if (remap < 0) remap = env_int("SBLAS_XCD_REMAP", 1);
+  sl.items = items;
+  sl.subwave = sub;
+  sl.nblocks = static_cast<int>(rblocks.size());
+  sl.xcd_remap = remap ? 1 : 0;
+  sl.grid = remap ? ((sl.nblocks + 7) / 8) * 8 : sl.nblocks;
+  int st;
+  if ((st = upload(*this, &d_rblocks, rblocks)) != SBLAS_OK) return st;
+  std::vector<ExecChunk> as_chunks;
+  as_chunks.reserve(rblocks.size());
+  for (const RowBlock &b : rblocks) as_chunks.push_back(ExecChunk{b.lo, b.hi, b.row_begin, b.row_end});
+  std::vector<int> mrows, mptr, mslots;
+  merge_lists(as_chunks, h_rp, mrows, mptr, mslots);
+  bgroups = static_cast<int>(mrows.size());
+  if ((st = upload(*this, &d_bmrows, mrows)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_bmptr, mptr)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_bmslots, mslots)) != SBLAS_OK) return st;
+  d_bpartials = static_cast<double *>(dev_alloc(sizeof(double) * 2 * std::max<size_t>(rblocks.size(), 1)));
+  if (!d_bpartials) {
+    set_error(SBLAS_ERR_ALLOC, "plan: partials allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  return SBLAS_OK;
+}
+
 hipEvent_t *Plan::timing_slot() {
   if (!timing) return nullptr;
   const int cap = 4096;
@@ -306,6 +380,51 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
   }
   int st = SBLAS_OK;
   const int C = static_cast<int>(exec.size());
+  if (uses_tiles(k)) {
+    // COO segmented reduction or merge-path CSR, then the ordered carry fix-up
+    hipEvent_t *te = timing_slot();
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
+    if (tiles == 0)
+      st = launch_scale(m, beta, Y, s);
+    else if (kernel == SBLAS_KERNEL_CSR5) {
+      st = launch_scale_rows(c5_empty, d_c5_empty, beta, Y, s);
+      if (st == SBLAS_OK)
+        st = launch_csr5(val_type, sigma, tiles, nnz, d_c5_tile_nz, d_c5_flags, d_c5_rows_nz,
+                         d_c5_colT, d_c5_valT, X, Y, alpha, beta, d_carry, s);
+    } else if (format == SBLAS_FMT_COO)
+      st = launch_coo_tiles(val_type, tile_sub, m, nnz, tiles, a0, a1, val, X, Y, alpha, beta,
+                            d_carry, s);
+    else
+      st = launch_csr_merge_path(val_type, tile_sub, m, nnz, tiles, a0, a1, val, X, Y, alpha,
+                                 beta, d_carry, s);
+    if (st != SBLAS_OK) return st;
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
+    launches++;
+    if (tiles > 0 && merge_enabled) {
+      st = launch_tile_fixup(tiles, d_carry, Y, alpha, beta, s);
+      launches++;
+    }
+    return st;
+  }
+  if (format == SBLAS_FMT_COO) {
+    set_error(SBLAS_ERR_UNSUPPORTED, "multi-vector products need a CSR or CSC plan");
+    return SBLAS_ERR_UNSUPPORTED;
+  }
+  if (variant == 2 && k == 1 && streams.empty()) {
+    hipEvent_t *te = timing_slot();
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
+    st = launch_csr_stream(val_type, sl, d_rblocks, a0, a1, val, X, Y, alpha, beta,
+                           d_bpartials, s);
+    if (st != SBLAS_OK) return st;
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
+    launches++;
+    if (bgroups > 0 && merge_enabled) {
+      st = launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, 1, Y, alpha, beta, s);
+      if (st != SBLAS_OK) return st;
+      launches++;
+    }
+    return SBLAS_OK;
+  }
   if (!streams.empty()) {
     // chunks over streams; the caller stream waits for all of them
     SBLAS_HIP_TRY(hipEventRecord(events[C], s));
@@ -336,12 +455,127 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
     launches++;
   }
-  if (ngroups > 0) {
+  if (ngroups > 0 && merge_enabled) {
     st = launch_merge(d_mrows, d_mptr, d_mslots, ngroups, d_partials, k, Y, alpha, beta, s);
     if (st != SBLAS_OK) return st;
     launches++;
   }
   return SBLAS_OK;
+}
+
+int Plan::setup_tiles() {
+  const int64_t items = (format == SBLAS_FMT_COO) ? nnz : static_cast<int64_t>(m) + nnz;
+  tiles = static_cast<int>((items + kTileSize - 1) / kTileSize);
+  tile_sub = opts.subwave > 0 ? opts.subwave : tile_subwave(stats.mean);
+  if (tile_sub != 1 && tile_sub != 4 && tile_sub != 16 && tile_sub != 64) tile_sub = 16;
+  d_carry = static_cast<TileCarry *>(dev_alloc(sizeof(TileCarry) * std::max(tiles, 1)));
+  if (!d_carry) {
+    set_error(SBLAS_ERR_ALLOC, "plan: tile carry allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  return SBLAS_OK;
+}
+
+// CSR5-style format: tiles of 64 lanes x sigma elements; per tile the index
+// of its first row among the non-empty rows (bit 31: the tile begins at a
+// row start); per lane a word of row-start bits; the element arrays copied
+// in tile-transposed order (padded to whole tiles).
+int Plan::setup_csr5(const std::vector<int> &h_rp) {
+  const int sg = opts.sigma > 0 ? opts.sigma : (stats.mean < 8 ? 8 : 16);
+  if (sg != 8 && sg != 16 && sg != 32) {
+    set_error(SBLAS_ERR_ARGS, "csr5: sigma must be 8, 16 or 32");
+    return SBLAS_ERR_ARGS;
+  }
+  sigma = sg;
+  const int64_t T = 64LL * sg;
+  tiles = static_cast<int>((nnz + T - 1) / T);
+  std::vector<int> rows_nz, empty, nzidx(static_cast<size_t>(std::max(m, 1)), -1);
+  for (int r = 0; r < m; r++) {
+    if (h_rp[r + 1] > h_rp[r]) {
+      nzidx[r] = static_cast<int>(rows_nz.size());
+      rows_nz.push_back(r);
+    } else {
+      empty.push_back(r);
+    }
+  }
+  std::vector<uint32_t> tile_nz(static_cast<size_t>(std::max(tiles, 1)), 0);
+  std::vector<uint32_t> flags(static_cast<size_t>(std::max(tiles, 1)) * 64, 0);
+  int r = 0;
+  for (int p = 0; p < tiles; p++) {
+    const int64_t e = static_cast<int64_t>(p) * T;
+    while (h_rp[r + 1] <= e) r++;
+    tile_nz[p] = static_cast<uint32_t>(nzidx[r]) | (h_rp[r] == e ? 0x80000000u : 0u);
+  }
+  for (int q : rows_nz) {
+    const int64_t e = h_rp[q];
+    const int64_t o = e % T;
+    if (o == 0) continue;  // tile start: bit 31 of the tile word
+    flags[static_cast<size_t>(e / T) * 64 + static_cast<size_t>(o / sg)] |= 1u << (o % sg);
+  }
+  c5_empty = static_cast<int>(empty.size());
+  int st;
+  if ((st = upload(*this, &d_c5_tile_nz, tile_nz)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_c5_flags, flags)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_c5_rows_nz, rows_nz)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_c5_empty, empty)) != SBLAS_OK) return st;
+  const int64_t padded = static_cast<int64_t>(tiles) * T;
+  const size_t vbytes = val_type == SBLAS_VAL_F64 ? 8 : (val_type == SBLAS_VAL_F32 ? 4 : 2);
+  d_c5_colT = static_cast<int *>(dev_alloc(sizeof(int) * std::max<int64_t>(padded, 1)));
+  d_c5_valT = dev_alloc(vbytes * std::max<int64_t>(padded, 1));
+  d_carry = static_cast<TileCarry *>(dev_alloc(sizeof(TileCarry) * std::max(tiles, 1)));
+  if (!d_c5_colT || !d_c5_valT || !d_carry) {
+    set_error(SBLAS_ERR_ALLOC, "csr5: format allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  if ((st = launch_csr5_transpose(val_type, sg, padded, nnz, a1, val, d_c5_colT, d_c5_valT,
+                                  nullptr)) != SBLAS_OK)
+    return st;
+  SBLAS_HIP_TRY(hipStreamSynchronize(nullptr));
+  return SBLAS_OK;
+}
+
+// Kernel choice and launch set-up for a CSR matrix whose row pointer is h_rp
+// (the caller's CSR, or the row-major form built from a CSC matrix).
+int Plan::finish_csr(const std::vector<int> &h_rp) {
+  stats = row_stats(m, h_rp.data());
+  kernel = opts.kernel != SBLAS_KERNEL_AUTO ? opts.kernel : autotune_kernel(stats);
+  if (kernel < SBLAS_KERNEL_VECTOR || kernel > SBLAS_KERNEL_CSR5) {
+    set_error(SBLAS_ERR_KERNEL, "CSR plans take kernel 1, 2 or 3");
+    return SBLAS_ERR_KERNEL;
+  }
+  // kernel 1 form: row blocks staged in LDS unless the sub-wave form is asked
+  // for; the sub-wave form also serves multi-RHS blocks and per-chunk streams
+  variant = (opts.variant == 1 || opts.streams > 1) ? 1 : 2;
+  if (opts.variant == 0) variant = env_int("SBLAS_VARIANT", variant);
+  int st;
+  if ((st = build_csr_chunks(h_rp.data())) != SBLAS_OK) return st;
+  if ((st = setup_vector(h_rp.data())) != SBLAS_OK) return st;
+  if (kernel == SBLAS_KERNEL_MERGE) return setup_tiles();
+  if (kernel == SBLAS_KERNEL_CSR5) return setup_csr5(h_rp);
+  if (variant == 2) return setup_stream(h_rp.data());
+  return SBLAS_OK;
+}
+
+bool Plan::uses_tiles(int k) const {
+  return format == SBLAS_FMT_COO ||
+         ((kernel == SBLAS_KERNEL_MERGE || kernel == SBLAS_KERNEL_CSR5) && k == 1);
+}
+
+bool Plan::has_merge(int k) const {
+  if (uses_tiles(k)) return tiles > 0;
+  if (variant == 2 && k == 1 && streams.empty()) return bgroups > 0;
+  return ngroups > 0;
+}
+
+int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) {
+  if (uses_tiles(k)) return tiles > 0 ? launch_tile_fixup(tiles, d_carry, Y, alpha, beta, s) : SBLAS_OK;
+  if (variant == 2 && k == 1 && streams.empty())
+    return bgroups > 0 ? launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, 1, Y,
+                                      alpha, beta, s)
+                       : SBLAS_OK;
+  return ngroups > 0
+             ? launch_merge(d_mrows, d_mptr, d_mslots, ngroups, d_partials, k, Y, alpha, beta, s)
+             : SBLAS_OK;
 }
 
 }  // namespace sblas
@@ -439,45 +673,136 @@ int sblas_plan_create_csr(int m, int n, int64_t nnz, const int *d_row_ptr, const
     *plan = nullptr;
     return SBLAS_ERR_HIP;
   }
+  auto fail = [&](int code) {
+    delete *plan;
+    *plan = nullptr;
+    return code;
+  };
+  for (int r = 0; r < m; r++)
+    if (h_rp[r + 1] < h_rp[r]) {
+      sblas::set_error(SBLAS_ERR_ARGS, "plan: row_ptr must be non-decreasing");
+      return fail(SBLAS_ERR_ARGS);
+    }
   if (h_rp[0] != 0 || h_rp[m] != nnz) {
     sblas::set_error(SBLAS_ERR_ARGS, "plan: row_ptr[0] must be 0 and row_ptr[m] == nnz");
-    delete *plan;
-    *plan = nullptr;
-    return SBLAS_ERR_ARGS;
+    return fail(SBLAS_ERR_ARGS);
   }
-  p.stats = sblas::row_stats(m, h_rp.data());
-  p.kernel = p.opts.kernel != SBLAS_KERNEL_AUTO ? p.opts.kernel : sblas::autotune_kernel(p.stats);
-  if (p.kernel == SBLAS_KERNEL_MERGE || p.kernel == SBLAS_KERNEL_CSR5) {
-    // until the tile kernels land every CSR plan runs the vector kernel
-    p.kernel = SBLAS_KERNEL_VECTOR;
+  bool ok = true;
+  if ((st = sblas::check_indices(d_col, nnz, 0, n, false, nullptr, &ok)) != SBLAS_OK)
+    return fail(st);
+  if (!ok) {
+    sblas::set_error(SBLAS_ERR_ARGS, "plan: a column index is outside [0, n)");
+    return fail(SBLAS_ERR_ARGS);
   }
-  if ((st = p.build_csr_chunks(h_rp.data())) != SBLAS_OK ||
-      (st = p.setup_vector(h_rp.data())) != SBLAS_OK) {
-    delete *plan;
-    *plan = nullptr;
-    return st;
-  }
+  if ((st = p.finish_csr(h_rp)) != SBLAS_OK) return fail(st);
   return SBLAS_OK;
 }
 
 int sblas_plan_create_coo(int m, int n, int64_t nnz, const int *d_row_idx, const int *d_col_idx,
                           const void *d_val, int val_type, const sblas_opts *opts,
                           sblas_plan *plan) {
-  (void)m; (void)n; (void)nnz; (void)d_row_idx; (void)d_col_idx; (void)d_val; (void)val_type;
-  (void)opts;
-  if (plan) *plan = nullptr;
-  sblas::set_error(SBLAS_ERR_UNSUPPORTED, "COO plans not built yet");
-  return SBLAS_ERR_UNSUPPORTED;
+  int st = plan_common(SBLAS_FMT_COO, m, n, nnz, d_row_idx, d_col_idx, d_val, val_type, opts,
+                       plan);
+  if (st != SBLAS_OK) return st;
+  Plan &p = (*plan)->impl;
+  auto fail = [&](int code) {
+    delete *plan;
+    *plan = nullptr;
+    return code;
+  };
+  bool ok_r = true, ok_c = true;
+  if ((st = sblas::check_indices(d_row_idx, nnz, 0, m, true, nullptr, &ok_r)) != SBLAS_OK ||
+      (st = sblas::check_indices(d_col_idx, nnz, 0, n, false, nullptr, &ok_c)) != SBLAS_OK)
+    return fail(st);
+  if (!ok_r || !ok_c) {
+    sblas::set_error(SBLAS_ERR_ARGS,
+                     "plan: COO row indices must be sorted in [0, m) and columns in [0, n)");
+    return fail(SBLAS_ERR_ARGS);
+  }
+  p.kernel = SBLAS_KERNEL_COO;
+  // descriptors of the COO rules need the row indices on the host:
+  // sblas_partition_v1_numa_coo / sblas_partition_baseline_coo give them
+  p.desc_status = SBLAS_ERR_UNSUPPORTED;
+  p.stats.mean = m > 0 ? static_cast<double>(nnz) / m : 0.0;
+  if ((st = p.setup_tiles()) != SBLAS_OK) return fail(st);
+  return SBLAS_OK;
 }
 
 int sblas_plan_create_csc(int m, int n, int64_t nnz, const int *d_col_ptr, const int *d_row_idx,
                           const void *d_val, int val_type, const sblas_opts *opts,
                           sblas_plan *plan) {
-  (void)m; (void)n; (void)nnz; (void)d_col_ptr; (void)d_row_idx; (void)d_val; (void)val_type;
-  (void)opts;
-  if (plan) *plan = nullptr;
-  sblas::set_error(SBLAS_ERR_UNSUPPORTED, "CSC plans not built yet");
-  return SBLAS_ERR_UNSUPPORTED;
+  int st = plan_common(SBLAS_FMT_CSC, m, n, nnz, d_col_ptr, d_row_idx, d_val, val_type, opts,
+                       plan);
+  if (st != SBLAS_OK) return st;
+  Plan &p = (*plan)->impl;
+  auto fail = [&](int code) {
+    delete *plan;
+    *plan = nullptr;
+    return code;
+  };
+  std::vector<int> h_cp(static_cast<size_t>(n) + 1);
+  if (hipMemcpy(h_cp.data(), d_col_ptr, sizeof(int) * h_cp.size(), hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    sblas::set_error(SBLAS_ERR_HIP, "plan: column pointer download failed");
+    return fail(SBLAS_ERR_HIP);
+  }
+  for (int c = 0; c < n; c++)
+    if (h_cp[c + 1] < h_cp[c]) {
+      sblas::set_error(SBLAS_ERR_ARGS, "plan: col_ptr must be non-decreasing");
+      return fail(SBLAS_ERR_ARGS);
+    }
+  if (h_cp[0] != 0 || h_cp[n] != nnz) {
+    sblas::set_error(SBLAS_ERR_ARGS, "plan: col_ptr[0] must be 0 and col_ptr[n] == nnz");
+    return fail(SBLAS_ERR_ARGS);
+  }
+  bool ok = true;
+  if ((st = sblas::check_indices(d_row_idx, nnz, 0, m, false, nullptr, &ok)) != SBLAS_OK)
+    return fail(st);
+  if (!ok) {
+    sblas::set_error(SBLAS_ERR_ARGS, "plan: a row index is outside [0, m)");
+    return fail(SBLAS_ERR_ARGS);
+  }
+  // column-partition descriptors of the reference's CSC rules
+  {
+    const int C = p.chunks;
+    p.descs.assign(C, sblas_chunk_desc{});
+    p.desc_status = (p.opts.part_mode == SBLAS_PART_ROWBLOCK)
+                        ? sblas::partition_rowblock(n, h_cp.data(), C, p.descs.data())
+                        : (nnz > 0 ? sblas::partition_v1_numa_csc(n, h_cp.data(), nnz, C, nullptr,
+                                                                   p.descs.data(), nullptr)
+                                   : SBLAS_ERR_UNDEFINED);
+  }
+  // row-major form, built once on the device
+  const size_t vbytes = val_type == SBLAS_VAL_F64 ? 8 : (val_type == SBLAS_VAL_F32 ? 4 : 2);
+  p.d_csr_rp = static_cast<int *>(p.dev_alloc(sizeof(int) * (static_cast<size_t>(m) + 1)));
+  p.d_csr_col = static_cast<int *>(p.dev_alloc(sizeof(int) * std::max<int64_t>(nnz, 1)));
+  p.d_csr_val = p.dev_alloc(vbytes * std::max<int64_t>(nnz, 1));
+  if (!p.d_csr_rp || !p.d_csr_col || !p.d_csr_val) {
+    sblas::set_error(SBLAS_ERR_ALLOC, "plan: CSC transpose allocation failed");
+    return fail(SBLAS_ERR_ALLOC);
+  }
+  if ((st = sblas::csc_to_csr_device(m, n, static_cast<int>(nnz), d_col_ptr, d_row_idx, d_val,
+                                     val_type, p.d_csr_rp, p.d_csr_col, p.d_csr_val, nullptr)) !=
+      SBLAS_OK)
+    return fail(st);
+  std::vector<int> h_rp(static_cast<size_t>(m) + 1);
+  if (hipMemcpy(h_rp.data(), p.d_csr_rp, sizeof(int) * h_rp.size(), hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    sblas::set_error(SBLAS_ERR_HIP, "plan: row pointer download failed");
+    return fail(SBLAS_ERR_HIP);
+  }
+  p.a0 = p.d_csr_rp;
+  p.a1 = p.d_csr_col;
+  p.val = p.d_csr_val;
+  const std::vector<sblas_chunk_desc> col_descs = p.descs;
+  const int col_status = p.desc_status;
+  sblas_opts o = p.opts;
+  p.opts.part_mode = SBLAS_PART_V1;  // the row-major form runs row chunks
+  if ((st = p.finish_csr(h_rp)) != SBLAS_OK) return fail(st);
+  p.opts = o;
+  p.descs = col_descs;
+  p.desc_status = col_status;
+  return SBLAS_OK;
 }
 
 int sblas_plan_spmv(sblas_plan plan, double alpha, const double *d_x, double beta, double *d_y,
@@ -520,6 +845,10 @@ int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info) {
   info->cv_row = p.stats.cv;
   info->max_row = p.stats.max_row;
   info->launches = p.launches;
+  info->variant = p.variant;
+  info->row_blocks = static_cast<int>(p.rblocks.size());
+  info->items = p.sl.items;
+  if (p.variant == 2) info->subwave = p.sl.subwave;
   return SBLAS_OK;
 }
 
@@ -529,6 +858,25 @@ int sblas_plan_get_chunks(sblas_plan plan, sblas_chunk_desc *out, int cap) {
   if (p.desc_status != SBLAS_OK) return -p.desc_status;
   const int n = static_cast<int>(p.descs.size());
   if (out) std::memcpy(out, p.descs.data(), sizeof(sblas_chunk_desc) * std::min(n, cap));
+  return n;
+}
+
+// Host-only view of the row blocks a plan would build (not in sblas.h; used
+// by the CPU tests to check the block invariants).  out holds 4 ints per
+// block: row_begin, row_end, lo, hi.  Returns the block count.
+int sblas_internal_row_blocks(int m, const int *rp, int64_t nnz, int C, int part_mode,
+                              int items, int *out, int cap) {
+  const std::vector<int64_t> bounds = sblas::element_bounds(part_mode, m, rp, nnz, C, 1);
+  const std::vector<sblas::ExecChunk> ex = sblas::exec_chunks_from_bounds(m, rp, bounds);
+  const std::vector<sblas::RowBlock> rb =
+      sblas::build_row_blocks(ex, rp, sblas::kStreamBlock * items);
+  const int n = static_cast<int>(rb.size());
+  for (int i = 0; i < std::min(n, cap); i++) {
+    out[4 * i] = rb[i].row_begin;
+    out[4 * i + 1] = rb[i].row_end;
+    out[4 * i + 2] = rb[i].lo;
+    out[4 * i + 3] = rb[i].hi;
+  }
   return n;
 }
 

@@ -16,6 +16,8 @@ RowStats row_stats(int m, const int *rp);
 int autotune_kernel(const RowStats &s);
 int autotune_subwave(const RowStats &s);
 int autotune_sigma(const RowStats &s);
+std::vector<RowBlock> build_row_blocks(const std::vector<ExecChunk> &ex, const int *rp,
+                                       int cap);
 const char *last_message();
 void clear_error();
 
@@ -51,12 +53,28 @@ struct Plan {
   double *d_partials = nullptr;       // 2 slots per chunk, x max_k
   int max_k = 16;
 
+  // row-block (LDS-staged) form of kernel 1
+  int variant = 1;                    // 1 sub-wave per row, 2 row blocks
+  std::vector<RowBlock> rblocks;
+  RowBlock *d_rblocks = nullptr;
+  StreamLaunch sl{};
+  int *d_bmrows = nullptr, *d_bmptr = nullptr, *d_bmslots = nullptr;
+  int bgroups = 0;
+  double *d_bpartials = nullptr;
+
   // segmented tiles (COO / merge path / CSR5)
   int tiles = 0;
   int sigma = 0;
   TileCarry *d_carry = nullptr;
   int *d_tile_row = nullptr;
   uint32_t *d_tile_flags = nullptr;
+
+  // CSR5-style format (kernel 3)
+  uint32_t *d_c5_tile_nz = nullptr, *d_c5_flags = nullptr;
+  int *d_c5_rows_nz = nullptr, *d_c5_empty = nullptr;
+  int *d_c5_colT = nullptr;
+  void *d_c5_valT = nullptr;
+  int c5_empty = 0;
 
   // CSC: row-major copy built once on the device (stable by-row transpose)
   int *d_csr_rp = nullptr, *d_csr_col = nullptr;
@@ -77,6 +95,15 @@ struct Plan {
   void *dev_alloc(size_t bytes);
   int build_csr_chunks(const int *h_rp);
   int setup_vector(const int *h_rp);
+  int setup_stream(const int *h_rp);
+  bool merge_enabled = true;          // false: caller launches run_merge itself
+  int tile_sub = 16;                  // lanes per row in the tile kernels
+  bool uses_tiles(int k) const;
+  bool has_merge(int k) const;
+  int setup_tiles();
+  int setup_csr5(const std::vector<int> &h_rp);
+  int finish_csr(const std::vector<int> &h_rp);
+  int run_merge(int k, double alpha, double beta, double *Y, hipStream_t s);
   hipEvent_t *timing_slot();
   int run_spmv(double alpha, const double *x, double beta, double *y, hipStream_t s);
   int run_spmm(int k, double alpha, const double *X, double beta, double *Y, hipStream_t s);

@@ -146,6 +146,104 @@ __global__ __launch_bounds__(BLOCK) void k_csr_vector_mrhs(
   }
 }
 
+// LDS-staged row blocks.  Phase 1 streams the block's element window with
+// consecutive threads on consecutive elements (every load instruction of a
+// wave covers whole cache lines; ITEMS loads of values and of columns are
+// issued back to back before any x gather), and stores the FP64 products in
+// LDS.  Phase 2 sums each row's products with SUBW lanes per row; phase 3
+// writes the block's y entries with consecutive threads on consecutive rows.
+// A block that holds a piece of one long row sums in registers and reduces
+// across the whole workgroup instead.
+template <typename VT, int ITEMS, int SUBW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_csr_stream(
+    const RowBlock *__restrict__ blocks, int nblocks, int xcd_remap,
+    const int *__restrict__ rp, const int *__restrict__ col, const VT *__restrict__ val,
+    const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
+    double *__restrict__ partials) {
+  constexpr int CAP = BLOCK * ITEMS;
+  __shared__ double prod[CAP];
+  __shared__ double ysh[kStreamRowsMax];
+  __shared__ double wsum[BLOCK / 64];
+  const int b = logical_block(nblocks, xcd_remap);
+  if (b < 0) return;
+  const RowBlock rb = blocks[b];
+  const int lo = rb.lo, hi = rb.hi;
+  const int t = threadIdx.x;
+
+  // phase 1: products of the window, ITEMS per thread, strided by BLOCK.
+  // Lanes past the window re-read its last element (same cache line) and
+  // their product is dropped by a select, so no load sits under a branch and
+  // all ITEMS value, column and x loads are in flight together.
+  double p[ITEMS];
+  if (hi > lo) {
+    double v[ITEMS];
+    int c[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+      const int e = min(lo + t + k * BLOCK, hi - 1);
+      v[k] = load_val(val, e);
+      c[k] = load_col(col, e);
+    }
+    double xv[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) xv[k] = x[c[k]];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) p[k] = (lo + t + k * BLOCK < hi) ? v[k] * xv[k] : 0.0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) p[k] = 0.0;
+  }
+
+  const int nrows = rb.row_end - rb.row_begin;
+  if (nrows == 1 && (rp[rb.row_begin] < lo || rp[rb.row_begin + 1] > hi)) {
+    // piece of a long row: register sum, wave reduction, then across waves
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) s += p[k];
+    s = subwave_sum<64>(s);
+    if ((t & 63) == 0) wsum[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+      double tot = 0.0;
+#pragma unroll
+      for (int w = 0; w < BLOCK / 64; w++) tot += wsum[w];
+      partials[2 * b] = tot;
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) {
+    const int e = t + k * BLOCK;
+    if (e < hi - lo) prod[e] = p[k];
+  }
+  __syncthreads();
+
+  // phase 2: one sub-wave of SUBW lanes per row
+  constexpr int SUBS = BLOCK / SUBW;
+  const int sub = t / SUBW, lane = t % SUBW;
+  for (int i = sub; i < nrows; i += SUBS) {
+    const int r = rb.row_begin + i;
+    const int s0 = max(rp[r], lo) - lo, e0 = min(rp[r + 1], hi) - lo;
+    double s = 0.0;
+    for (int j = s0 + lane; j < e0; j += SUBW) s += prod[j];
+    if constexpr (SUBW > 1) s = subwave_sum<SUBW>(s);
+    if (lane == 0) ysh[i] = s;
+  }
+  __syncthreads();
+
+  // phase 3: y for whole rows, partials for rows cut by the window
+  for (int i = t; i < nrows; i += BLOCK) {
+    const int r = rb.row_begin + i;
+    const double s = ysh[i];
+    const bool edge = (i == 0 || i == nrows - 1);
+    if (edge && (rp[r] < lo || rp[r + 1] > hi)) {
+      partials[2 * b + (i == 0 ? 0 : 1)] = s;
+    } else {
+      y[r] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[r];
+    }
+  }
+}
+
 // Split rows: y[row] = alpha * (partials of the row in chunk order) + beta*y.
 __global__ void k_merge(const int *__restrict__ rows, const int *__restrict__ ptr,
                         const int *__restrict__ slots, int groups,
@@ -158,6 +256,19 @@ __global__ void k_merge(const int *__restrict__ rows, const int *__restrict__ pt
   for (int q = ptr[g]; q < ptr[g + 1]; q++) s += partials[slots[q] * k + v];
   double *yp = y + static_cast<int64_t>(rows[g]) * k + v;
   *yp = (beta == 0.0) ? alpha * s : alpha * s + beta * (*yp);
+}
+
+// Index validation at plan time: flag[0] = 1 when some a[i] is outside
+// [lo, hi) or, with `sorted`, a[i] > a[i+1].  Every offending thread stores
+// the same value.
+__global__ void k_check_indices(int64_t count, const int *__restrict__ a, int lo, int hi,
+                                int sorted, int *__restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int v = a[i];
+  bool bad = v < lo || v >= hi;
+  if (sorted && i + 1 < count) bad = bad || v > a[i + 1];
+  if (bad) flag[0] = 1;
 }
 
 __global__ void k_scale(int64_t count, double beta, double *__restrict__ y) {
@@ -240,7 +351,70 @@ int launch_mrhs_typed(int k, const VectorLaunch &L, const DevChunk *chunks, int 
   }
 }
 
+template <typename VT, int ITEMS>
+int launch_stream_items(const StreamLaunch &L, const RowBlock *blocks, const int *rp,
+                        const int *col, const VT *val, const double *x, double *y,
+                        double alpha, double beta, double *partials, hipStream_t s) {
+  constexpr int B = kStreamBlock;
+  const dim3 grid(L.grid), block(B);
+#define SBLAS_STREAM_CASE(W)                                                              \
+  case W:                                                                                 \
+    hipLaunchKernelGGL((k_csr_stream<VT, ITEMS, W, B>), grid, block, 0, s, blocks,        \
+                       L.nblocks, L.xcd_remap, rp, col, val, x, y, alpha, beta, partials); \
+    break;
+  switch (L.subwave) {
+    SBLAS_STREAM_CASE(1)
+    SBLAS_STREAM_CASE(2)
+    SBLAS_STREAM_CASE(4)
+    SBLAS_STREAM_CASE(8)
+    SBLAS_STREAM_CASE(16)
+    SBLAS_STREAM_CASE(32)
+    SBLAS_STREAM_CASE(64)
+    default:
+      set_error(SBLAS_ERR_ARGS, "stream kernel: sub-wave must be a power of two in [1,64]");
+      return SBLAS_ERR_ARGS;
+  }
+#undef SBLAS_STREAM_CASE
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+template <typename VT>
+int launch_stream_typed(const StreamLaunch &L, const RowBlock *blocks, const int *rp,
+                        const int *col, const VT *val, const double *x, double *y,
+                        double alpha, double beta, double *partials, hipStream_t s) {
+  switch (L.items) {
+    case 8:
+      return launch_stream_items<VT, 8>(L, blocks, rp, col, val, x, y, alpha, beta, partials, s);
+    case 16:
+      return launch_stream_items<VT, 16>(L, blocks, rp, col, val, x, y, alpha, beta, partials, s);
+    default:
+      set_error(SBLAS_ERR_ARGS, "stream kernel: items per thread must be 8 or 16");
+      return SBLAS_ERR_ARGS;
+  }
+}
+
 }  // namespace
+
+int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *blocks,
+                      const int *rp, const int *col, const void *val, const double *x,
+                      double *y, double alpha, double beta, double *partials, hipStream_t s) {
+  if (L.grid <= 0) return SBLAS_OK;
+  switch (val_type) {
+    case SBLAS_VAL_F64:
+      return launch_stream_typed(L, blocks, rp, col, static_cast<const double *>(val), x, y,
+                                 alpha, beta, partials, s);
+    case SBLAS_VAL_F32:
+      return launch_stream_typed(L, blocks, rp, col, static_cast<const float *>(val), x, y,
+                                 alpha, beta, partials, s);
+    case SBLAS_VAL_BF16:
+      return launch_stream_typed(L, blocks, rp, col, static_cast<const uint16_t *>(val), x, y,
+                                 alpha, beta, partials, s);
+    default:
+      set_error(SBLAS_ERR_ARGS, "unknown value type");
+      return SBLAS_ERR_ARGS;
+  }
+}
 
 int launch_csr_vector(int val_type, const VectorLaunch &L, const DevChunk *chunks, int nchunks,
                       const int *rp, const int *col, const void *val, const double *x,
@@ -294,6 +468,30 @@ int launch_merge(const int *rows, const int *ptr, const int *slots, int groups,
   hipLaunchKernelGGL(k_merge, dim3((threads + 255) / 256), dim3(256), 0, s, rows, ptr, slots,
                      groups, partials, k, y, alpha, beta);
   SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+int check_indices(const int *a, int64_t count, int lo, int hi, bool sorted, hipStream_t s,
+                  bool *ok) {
+  *ok = true;
+  if (count <= 0) return SBLAS_OK;
+  int *flag = nullptr;
+  SBLAS_HIP_TRY(hipMalloc(&flag, sizeof(int)));
+  int h = 0;
+  hipError_t e = hipMemcpyAsync(flag, &h, sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_check_indices, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256),
+                       0, s, count, a, lo, hi, sorted ? 1 : 0, flag);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(flag);
+  if (e != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, std::string("index check: ") + hipGetErrorString(e));
+    return SBLAS_ERR_HIP;
+  }
+  *ok = (h == 0);
   return SBLAS_OK;
 }
 

@@ -36,7 +36,7 @@ def val_type_of(t) -> int:
 
 
 def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=0, sigma=0,
-              xcd_remap=-1) -> Opts:
+              xcd_remap=-1, variant=0, items=0) -> Opts:
     o = Opts()
     lib().sblas_opts_default(ctypes.byref(o))
     o.kernel = _KERNELS[kernel]
@@ -47,6 +47,8 @@ def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=
     o.subwave = subwave
     o.sigma = sigma
     o.xcd_remap = xcd_remap
+    o.variant = variant
+    o.items = items
     return o
 
 

@@ -171,7 +171,8 @@ enum {
   SBLAS_KERNEL_AUTO = 0,
   SBLAS_KERNEL_VECTOR = 1,  /* vector CSR, wave64 split into sub-waves      */
   SBLAS_KERNEL_MERGE = 2,   /* merge-path CSR                               */
-  SBLAS_KERNEL_CSR5 = 3     /* CSR5-style tiles, wave64 segmented sum       */
+  SBLAS_KERNEL_CSR5 = 3,    /* CSR5-style tiles, wave64 segmented sum       */
+  SBLAS_KERNEL_COO = 4      /* COO segmented reduction (COO plans)          */
 };
 enum {
   SBLAS_PART_V1 = 0,        /* src/dspmv_mgpu_v1.cu:119-130                 */
@@ -189,7 +190,10 @@ typedef struct sblas_opts {
   int subwave;     /* vector kernel lanes per row (0 = from row lengths)    */
   int sigma;       /* CSR5 nnz per lane per tile (0 = autotuned)            */
   int xcd_remap;   /* 1: contiguous row ranges per XCD; 0: off; -1: auto   */
-  int reserved[8];
+  int variant;     /* kernel 1 form: 0 auto, 1 sub-wave per row, 2 row blocks */
+                   /* staged in LDS                                          */
+  int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
+  int reserved[6];
 } sblas_opts;
 
 /* Fills opts with defaults (auto kernel, C=1, v1 partition, fused launch). */
@@ -224,7 +228,10 @@ typedef struct sblas_plan_info {
   double mean_row, cv_row;
   int64_t max_row;
   int launches;      /* kernel launches per sblas_plan_spmv                 */
-  int reserved[7];
+  int variant;       /* kernel 1 form in use (1 sub-wave, 2 row blocks)      */
+  int row_blocks;    /* row blocks of the row-block form                    */
+  int items;         /* elements per thread of the row-block form           */
+  int reserved[4];
 } sblas_plan_info;
 int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info);
 /* Chunk descriptors of the plan (reference rule of opts.part_mode). */

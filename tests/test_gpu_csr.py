@@ -43,7 +43,7 @@ def _csc_of(m, n, rp, col, val):
     return G.to_csc(A)
 
 
-ENTRY_POINTS = ["v1_k1", "baseline", "v1_numa", "v2", "baseline_coo", "v1_numa_coo",
+ENTRY_POINTS = ["v1_k1", "v1_k2", "v1_k3", "baseline", "v1_numa", "v2", "baseline_coo", "v1_numa_coo",
                 "baseline_csc", "v1_numa_csc"]
 
 
@@ -270,9 +270,10 @@ def test_plan_streams_mode_matches_fused(oracle):
     x = G.values(A.n, 0, 2, "uniform")
     rp, col, val, dx = _dev(A.row_ptr, A.col, A.val, x)
     outs = []
+    # same sub-wave kernel launched fused (one launch) and per chunk on 4 streams
     for streams in (1, 4):
         plan = SpmvPlan("csr", A.m, A.n, rp, col, val, kernel="vector", chunks=8,
-                        streams=streams)
+                        streams=streams, variant=1)
         dy = torch.zeros(A.m, dtype=torch.float64, device="cuda")
         plan.spmv(dx, dy)
         torch.cuda.synchronize()
@@ -281,3 +282,10 @@ def test_plan_streams_mode_matches_fused(oracle):
     assert np.array_equal(outs[0], outs[1])
     ref = oracle.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, 1.0, 0.0, np.zeros(A.m))
     assert oracle.err_normwise(outs[0], ref) <= TOL
+    # the row-block form (default for fused launches) over the same chunks
+    plan = SpmvPlan("csr", A.m, A.n, rp, col, val, kernel="vector", chunks=8)
+    assert plan.info()["variant"] == 2
+    dy = torch.zeros(A.m, dtype=torch.float64, device="cuda")
+    plan.spmv(dx, dy)
+    torch.cuda.synchronize()
+    assert oracle.err_normwise(dy.cpu().numpy(), ref) <= TOL
