@@ -105,6 +105,169 @@ def cpu_baseline(A, x, y0, alpha, beta, budget_s=20.0):
     }, y
 
 
+def _time_calls(fn, reps=20, warmup=3):
+    """Median wall time (ms) of fn() on the current stream, by torch events
+    around each call (fn launches on torch's current stream)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        ts.append((e0, e1))
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in ts]))
+
+
+def extras_rmat(log_to):
+    """R-MAT scale 22 (edge factor 17, binary values, alpha 1, beta 0.5): FP64
+    kernels 1/2/3, COO and CSC forms; FP32 / BF16 values and a k=8 block of
+    right-hand sides, each with its error against the FP64 oracle."""
+    import torch
+    from hpda_sblas_pnnl_amd import gen as G
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    from oracle import oracle as ora
+    dev = torch.device("cuda:0")
+    A = G.rmat_csr(scale=22, edge_factor=17, seed=0)
+    x = G.values(A.n, 0, 2, "uniform")
+    y0 = G.values(A.m, 0, 3, "uniform")
+    alpha, beta = 1.0, 0.5
+    cores = len(os.sched_getaffinity(0))
+    ref = ora.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, alpha, beta, y0, threads=cores)
+    rp, col, val = (torch.from_numpy(v).to(dev) for v in (A.row_ptr, A.col, A.val))
+    dx, dy0 = torch.from_numpy(x).to(dev), torch.from_numpy(y0).to(dev)
+    out = {"workload": "R-MAT scale 22, edge factor 17, dedup, binary values, beta 0.5",
+           "m": A.m, "nnz": A.nnz}
+
+    def run(plan, fmt_bytes_val=8, fmt="csr"):
+        dy = dy0.clone()
+        plan.spmv(dx, dy, alpha, beta)
+        torch.cuda.synchronize()
+        err = ora.err_normwise(dy.cpu().numpy(), ref)
+        maxrow = ora.err_maxrow(A.m, A.val, A.row_ptr, A.col, x, alpha, beta, y0,
+                                dy.cpu().numpy(), ref)
+        ms = _time_calls(lambda: plan.spmv(dx, dy, alpha, beta))
+        cb = compulsory_bytes(A.m, A.n, A.nnz, fmt_bytes_val, 1, True)
+        if fmt == "coo":
+            cb += A.nnz * 4 - (A.m + 1) * 4
+        return {"ms": round(ms, 4), "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
+                "GBs": round(cb / ms / 1e6, 1), "frac": round(cb / ms / 1e6 / PEAK_HBM_GBS, 4),
+                "err_normwise": err, "err_maxrow": maxrow}
+
+    kernels = {}
+    for name, kw in (("kernel1_rowblock", {"kernel": "vector"}), ("kernel2_merge", {"kernel": "merge"}),
+                     ("kernel3_csr5", {"kernel": "csr5"})):
+        p = SpmvPlan("csr", A.m, A.n, rp, col, val, **kw)
+        kernels[name] = run(p)
+        del p
+        log_to(f"[bench] rmat {name}: {kernels[name]}")
+    ri = torch.from_numpy(G.coo_rows(A)).to(dev)
+    p = SpmvPlan("coo", A.m, A.n, ri, col, val)
+    kernels["coo_segmented"] = run(p, fmt="coo")
+    del p, ri
+    cp, rows, cv = G.to_csc(A)
+    dcp, drows, dcv = (torch.from_numpy(v).to(dev) for v in (cp, rows, cv))
+    p = SpmvPlan("csc", A.m, A.n, dcp, drows, dcv)
+    kernels["csc_transposed"] = run(p)
+    del p, dcp, drows, dcv
+    out["fp64"] = kernels
+    base_ms = kernels["kernel1_rowblock"]["ms"]
+
+    prec = {}
+    v32 = torch.from_numpy(A.val.astype(np.float32)).to(dev)
+    p = SpmvPlan("csr", A.m, A.n, rp, col, v32)
+    prec["fp32_values"] = run(p, 4)
+    del p
+    v16 = torch.from_numpy(G.to_bf16_bits(A.val).view(np.int16)).to(dev)
+    p = SpmvPlan("csr", A.m, A.n, rp, col, v16)
+    prec["bf16_values"] = run(p, 2)
+    del p
+    for k in prec:
+        prec[k]["speedup_vs_fp64"] = round(base_ms / prec[k]["ms"], 3)
+    # k = 8 right-hand sides, X row-major n x 8
+    K = 8
+    X = G.values(A.n * K, 0, 4, "uniform").reshape(A.n, K)
+    Y0 = G.values(A.m * K, 0, 5, "uniform").reshape(A.m, K)
+    dX, dY0 = torch.from_numpy(X).to(dev), torch.from_numpy(Y0).to(dev)
+    p = SpmvPlan("csr", A.m, A.n, rp, col, val)
+    dY = dY0.clone()
+    p.spmm(dX, dY, alpha, beta)
+    torch.cuda.synchronize()
+    Yref = ora.csr_spmm(A.m, K, A.val, A.row_ptr, A.col, X, alpha, beta, Y0)
+    err = ora.err_normwise(dY.cpu().numpy().ravel(), Yref.ravel())
+    ms = _time_calls(lambda: p.spmm(dX, dY, alpha, beta))
+    prec["spmm_k8_fp64"] = {"ms": round(ms, 4), "ms_per_vector": round(ms / K, 4),
+                            "speedup_per_vector_vs_fp64": round(base_ms * K / ms, 3),
+                            "GFLOPs": round(2 * A.nnz * K / ms / 1e6, 1), "err_normwise": err}
+    del p, dX, dY, dY0
+    out["precision"] = prec
+    return out
+
+
+def extras_batched(log_to, count=64):
+    """64 matrices (log-uniform n in [1e4, 2e6], half banded, half R-MAT),
+    FP32 values with FP64 accumulation, one persistent launch replayed as a
+    hipGraph; against the same matrices run one FP64 plan at a time."""
+    import torch
+    from hpda_sblas_pnnl_amd import gen as G
+    from hpda_sblas_pnnl_amd.device import SpmvBatch, SpmvPlan
+    from oracle import oracle as ora
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    mats = G.batched_workload(count, seed=0)
+    total = sum(A.nnz for _, A in mats)
+    log_to(f"[bench] batched: {count} matrices, nnz={total} generated in {time.time() - t0:.1f}s")
+    cores = len(os.sched_getaffinity(0))
+    items, plans, xs, refs, y64 = [], [], [], [], []
+    for i, (kind, A) in enumerate(mats):
+        x = G.values(A.n, i, 2, "uniform")
+        dx = torch.from_numpy(x).to(dev)
+        rp = torch.from_numpy(A.row_ptr).to(dev)
+        col = torch.from_numpy(A.col).to(dev)
+        v32 = torch.from_numpy(A.val.astype(np.float32)).to(dev)
+        v64 = torch.from_numpy(A.val).to(dev)
+        items.append(dict(m=A.m, n=A.n, row_ptr=rp, col=col, val=v32, x=dx,
+                          y=torch.zeros(A.m, dtype=torch.float64, device=dev), alpha=1.0, beta=0.0))
+        plans.append((SpmvPlan("csr", A.m, A.n, rp, col, v64), dx,
+                      torch.zeros(A.m, dtype=torch.float64, device=dev), v64))
+        refs.append(ora.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, 1.0, 0.0, np.zeros(A.m),
+                                 threads=cores))
+    batch = SpmvBatch(items)
+    batch.graph_launch()
+    torch.cuda.synchronize()
+    errs = [ora.err_normwise(it["y"].cpu().numpy(), r) for it, r in zip(items, refs)]
+    ms_graph = _time_calls(batch.graph_launch, reps=10)
+
+    def fp64_all():
+        for p, dx, dy, _ in plans:
+            p.spmv(dx, dy)
+    ms_fp64 = _time_calls(fp64_all, reps=10)
+    errs64 = []
+    for (p, dx, dy, _), r in zip(plans, refs):
+        errs64.append(ora.err_normwise(dy.cpu().numpy(), r))
+    choices = [batch.choice(i) for i in range(count)]
+    cb32 = sum(compulsory_bytes(A.m, A.n, A.nnz, 4) for _, A in mats)
+    out = {
+        "workload": f"{count} matrices, n log-uniform in [1e4, 2e6] (seed 0), even banded "
+                    "(half-width min(50000, n/40)), odd R-MAT edge factor 16 cut to n; "
+                    "FP32 values, FP64 accumulation",
+        "matrices": count, "nnz_total": total,
+        "graph_ms": round(ms_graph, 3), "fp64_separate_ms": round(ms_fp64, 3),
+        "speedup_vs_fp64_separate": round(ms_fp64 / ms_graph, 3),
+        "GFLOPs": round(2 * total / ms_graph / 1e6, 1),
+        "GBs": round(cb32 / ms_graph / 1e6, 1),
+        "max_err_normwise": max(errs), "max_err_normwise_fp64_path": max(errs64),
+        "shape": batch.shape(),
+        "autotuner_subwaves": sorted({c["subwave"] for c in choices}),
+    }
+    return out
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
@@ -126,6 +289,10 @@ def main():
     ap.add_argument("--subwave", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the R-MAT precision and batched measurements")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 1 GiB device copy first (reference for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -138,6 +305,15 @@ def main():
 
     assert torch.cuda.is_available(), "no HIP device"
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        # a plain copy of known size: reads 1 GiB, writes 1 GiB
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 1073741824 bytes read, 1073741824 bytes written")
+        del a, b
     A, x, y0, alpha, beta, chunks, part, desc = make_workload(args.workload)
     if args.chunks:
         chunks = args.chunks
@@ -205,6 +381,19 @@ def main():
         from oracle import oracle as ora
         line["cpu_baseline"] = cpu
         line["err_normwise_vs_oracle"] = ora.err_normwise(y_gpu, y_ref)
+    if not args.no_extras and args.workload == "banded":
+        del plan, rp, col, val
+        torch.cuda.empty_cache()
+        extras = {}
+        for name, fn in (("rmat", extras_rmat), ("batched", extras_batched)):
+            t0 = time.time()
+            try:
+                extras[name] = fn(log)
+            except Exception as e:  # noqa: BLE001  (the main line must still print)
+                extras[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] extras {name} done in {time.time() - t0:.1f}s")
+            torch.cuda.empty_cache()
+        line["extras"] = extras
     print(json.dumps(line), flush=True)
 
 

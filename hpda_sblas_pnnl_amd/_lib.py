@@ -121,7 +121,28 @@ SIGNATURES = {
     "sblas_plan_destroy": (_I, [_P]),
     "sblas_plan_set_timing": (_I, [_P, _I]),
     "sblas_plan_kernel_times": (_I, [_P, _P, _I]),
+    "sblas_batch_create": (_I, [_I, _P, ctypes.POINTER(Opts), ctypes.POINTER(_P)]),
+    "sblas_batch_run": (_I, [_P, _P]),
+    "sblas_batch_graph_launch": (_I, [_P, _P]),
+    "sblas_batch_get_choice": (_I, [_P, _I, _P]),
+    "sblas_batch_get_shape": (_I, [_P, _P, _P, _P]),
+    "sblas_batch_destroy": (_I, [_P]),
 }
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("m", _I), ("n", _I), ("nnz", _I64), ("d_row_ptr", _P), ("d_col", _P),
+                ("d_val", _P), ("val_type", _I), ("pad_", _I), ("d_x", _P), ("d_y", _P),
+                ("alpha", _D), ("beta", _D)]
+
+
+class BatchChoice(ctypes.Structure):
+    _fields_ = [("kernel", _I), ("subwave", _I), ("items", _I), ("chunks", _I),
+                ("row_blocks", _I), ("pad_", _I), ("mean_row", _D), ("cv_row", _D),
+                ("max_row", _I64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
 
 _lib = None
 _gen = None
@@ -168,6 +189,9 @@ def gen() -> ctypes.CDLL:
         G.sblas_gen_rmat.restype = _I64
         G.sblas_gen_rmat.argtypes = [_I, _I64, _D, _D, _D, ctypes.c_uint64,
                                      ctypes.POINTER(_IP), ctypes.POINTER(_IP)]
+        G.sblas_gen_rmat_n.restype = _I64
+        G.sblas_gen_rmat_n.argtypes = [_I, _I64, _D, _D, _D, ctypes.c_uint64, _I64,
+                                       ctypes.POINTER(_IP), ctypes.POINTER(_IP)]
         G.sblas_gen_free.argtypes = [_P]
         G.sblas_gen_free.restype = None
         G.sblas_gen_csr_to_csc.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P]

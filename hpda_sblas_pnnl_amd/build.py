@@ -27,6 +27,8 @@ HIP_SOURCES = [
     "sblas_tiles.hip",
     "sblas_csc.hip",
     "sblas_csr5.hip",
+    "sblas_batch.hip",
+    "batch.cpp",
     "plan.cpp",
     "dropin.cpp",
     "partition.cpp",

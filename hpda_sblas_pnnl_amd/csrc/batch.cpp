@@ -1,0 +1,275 @@
+// batch.cpp -- batched SpMV over many CSR matrices in one persistent launch,
+// with a per-matrix autotuner and hipGraph replay.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+
+#include "plan.hpp"
+
+namespace sblas {
+namespace {
+
+// Autotuner decision for one row-length profile.  Lanes per row follow the
+// mean row length (a quarter of it, so a sub-wave strides each row about
+// four times through LDS); matrices above 2^24 elements are cut into 8 nnz
+// chunks (v1 rule) before packing row blocks, so their work items
+// interleave by chunk in the list; smaller ones keep one chunk.  Decisions
+// are cached by the rounded profile so equal-shaped matrices agree.
+struct Choice {
+  int subwave = 16, items = 8, chunks = 1;
+};
+
+int pow2_round_b(double v, int lo, int hi) {
+  if (!(v > 0)) return lo;
+  const int p = 1 << static_cast<int>(std::lround(std::log2(v)));
+  return std::min(std::max(p, lo), hi);
+}
+
+Choice decide(const RowStats &s, int64_t nnz) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, Choice> cache;
+  const std::pair<int, int> key{static_cast<int>(std::lround(std::log2(std::max(1.0, s.mean)) * 4)),
+                                nnz > (int64_t{1} << 24) ? 1 : 0};
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  Choice c;
+  c.subwave = pow2_round_b(s.mean / 4.0, 1, 64);
+  c.items = 8;
+  c.chunks = key.second ? 8 : 1;
+  cache[key] = c;
+  return c;
+}
+
+}  // namespace
+}  // namespace sblas
+
+struct sblas_batch_s {
+  std::vector<sblas::BatchMatrix> mats;
+  std::vector<sblas::BatchItem> items;
+  std::vector<sblas_batch_choice> choices;
+  sblas::BatchMatrix *d_mats = nullptr;
+  sblas::BatchItem *d_items = nullptr;
+  double *d_partials = nullptr;
+  int *d_gmat = nullptr, *d_grow = nullptr, *d_gptr = nullptr, *d_gslot = nullptr;
+  int groups = 0;
+  int grid = 0;
+  int items_per_thread = 8;
+  std::vector<void *> bufs;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  ~sblas_batch_s() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    for (void *p : bufs) (void)hipFree(p);
+  }
+  template <typename T>
+  int upload(T **dst, const std::vector<T> &src) {
+    void *p = nullptr;
+    const size_t bytes = sizeof(T) * std::max<size_t>(src.size(), 1);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      sblas::set_error(SBLAS_ERR_ALLOC, "batch: device allocation failed");
+      return SBLAS_ERR_ALLOC;
+    }
+    bufs.push_back(p);
+    *dst = static_cast<T *>(p);
+    if (!src.empty() &&
+        hipMemcpy(p, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      sblas::set_error(SBLAS_ERR_HIP, "batch: upload failed");
+      return SBLAS_ERR_HIP;
+    }
+    return SBLAS_OK;
+  }
+  int enqueue(hipStream_t s) {
+    int st = sblas::launch_batch_rowblocks(items_per_thread, grid, static_cast<int>(items.size()),
+                                           d_items, d_mats, d_partials, s);
+    if (st != SBLAS_OK) return st;
+    return sblas::launch_batch_merge(groups, d_gmat, d_grow, d_gptr, d_gslot, d_mats, d_partials,
+                                     s);
+  }
+};
+
+extern "C" {
+
+int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opts *opts,
+                       sblas_batch *out) {
+  using namespace sblas;
+  clear_error();
+  if (!out || count < 0 || (count > 0 && !items)) {
+    set_error(SBLAS_ERR_ARGS, "batch: bad arguments");
+    return SBLAS_ERR_ARGS;
+  }
+  *out = nullptr;
+  sblas_batch b = new (std::nothrow) sblas_batch_s;
+  if (!b) {
+    set_error(SBLAS_ERR_ALLOC, "batch: host allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  auto fail = [&](int code) {
+    delete b;
+    return code;
+  };
+  std::vector<int> gmat, grow, gptr(1, 0), gslot;
+  for (int i = 0; i < count; i++) {
+    const sblas_batch_item &it = items[i];
+    if (it.m < 0 || it.n < 0 || it.nnz < 0 || it.nnz > INT32_MAX - 4096 || !it.d_row_ptr ||
+        (it.nnz > 0 && (!it.d_col || !it.d_val || !it.d_x)) || (it.m > 0 && !it.d_y) ||
+        it.val_type < SBLAS_VAL_F64 || it.val_type > SBLAS_VAL_BF16) {
+      set_error(SBLAS_ERR_ARGS, "batch: bad item " + std::to_string(i));
+      return fail(SBLAS_ERR_ARGS);
+    }
+    std::vector<int> h_rp(static_cast<size_t>(it.m) + 1);
+    if (hipMemcpy(h_rp.data(), it.d_row_ptr, sizeof(int) * h_rp.size(), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      set_error(SBLAS_ERR_HIP, "batch: row pointer download failed");
+      return fail(SBLAS_ERR_HIP);
+    }
+    if (h_rp[0] != 0 || h_rp[it.m] != it.nnz) {
+      set_error(SBLAS_ERR_ARGS, "batch: row_ptr[0] must be 0 and row_ptr[m] == nnz");
+      return fail(SBLAS_ERR_ARGS);
+    }
+    for (int r = 0; r < it.m; r++)
+      if (h_rp[r + 1] < h_rp[r]) {
+        set_error(SBLAS_ERR_ARGS, "batch: row_ptr must be non-decreasing");
+        return fail(SBLAS_ERR_ARGS);
+      }
+    bool ok = true;
+    int st = check_indices(it.d_col, it.nnz, 0, it.n, false, nullptr, &ok);
+    if (st != SBLAS_OK) return fail(st);
+    if (!ok) {
+      set_error(SBLAS_ERR_ARGS, "batch: column index outside [0, n) in item " + std::to_string(i));
+      return fail(SBLAS_ERR_ARGS);
+    }
+    const RowStats rs = row_stats(it.m, h_rp.data());
+    Choice c = decide(rs, it.nnz);
+    if (opts && opts->subwave > 0) c.subwave = opts->subwave;
+    if (opts && opts->chunks > 0) c.chunks = opts->chunks;
+    b->items_per_thread = (opts && opts->items == 16) ? 16 : 8;
+    c.items = b->items_per_thread;
+
+    BatchMatrix M{};
+    M.rp = it.d_row_ptr;
+    M.col = it.d_col;
+    M.val = it.d_val;
+    M.x = it.d_x;
+    M.y = it.d_y;
+    M.alpha = it.alpha;
+    M.beta = it.beta;
+    M.val_type = it.val_type;
+    M.subwave = c.subwave;
+    M.m = it.m;
+    b->mats.push_back(M);
+
+    if (it.m == 0) {
+      b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks, 0,
+                                              0, rs.mean, rs.cv, rs.max_row});
+      continue;
+    }
+    const std::vector<int64_t> bounds =
+        element_bounds(SBLAS_PART_V1, it.m, h_rp.data(), it.nnz, c.chunks, 1);
+    const std::vector<ExecChunk> ex = exec_chunks_from_bounds(it.m, h_rp.data(), bounds);
+    const std::vector<RowBlock> rb = build_row_blocks(ex, h_rp.data(), kStreamBlock * c.items);
+    const int first_item = static_cast<int>(b->items.size());
+    for (const RowBlock &r : rb)
+      b->items.push_back(BatchItem{i, r.row_begin, r.row_end, r.lo, r.hi, {0, 0, 0}});
+    // split rows of this matrix: partial slots 2*item (+1 for a block's last row)
+    for (size_t q = 0; q < rb.size(); q++) {
+      const RowBlock &r = rb[q];
+      const int item = first_item + static_cast<int>(q);
+      auto split = [&](int row) { return h_rp[row] < r.lo || h_rp[row + 1] > r.hi; };
+      auto add = [&](int row, int slot) {
+        if (!grow.empty() && gmat.back() == i && grow.back() == row) {
+          gslot.push_back(slot);
+          gptr.back() = static_cast<int>(gslot.size());
+        } else {
+          gmat.push_back(i);
+          grow.push_back(row);
+          gslot.push_back(slot);
+          gptr.push_back(static_cast<int>(gslot.size()));
+        }
+      };
+      const int first = r.row_begin, last = r.row_end - 1;
+      if (split(first)) add(first, 2 * item);
+      if (last != first && split(last)) add(last, 2 * item + 1);
+    }
+    b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks,
+                                            static_cast<int>(rb.size()), 0, rs.mean, rs.cv,
+                                            rs.max_row});
+  }
+  b->groups = static_cast<int>(grow.size());
+  int st;
+  if ((st = b->upload(&b->d_mats, b->mats)) != SBLAS_OK ||
+      (st = b->upload(&b->d_items, b->items)) != SBLAS_OK ||
+      (st = b->upload(&b->d_gmat, gmat)) != SBLAS_OK ||
+      (st = b->upload(&b->d_grow, grow)) != SBLAS_OK ||
+      (st = b->upload(&b->d_gptr, gptr)) != SBLAS_OK ||
+      (st = b->upload(&b->d_gslot, gslot)) != SBLAS_OK)
+    return fail(st);
+  std::vector<double> zeros(2 * std::max<size_t>(b->items.size(), 1), 0.0);
+  if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  // a few resident workgroups per CU, never more than there are items
+  b->grid = std::min<int>(static_cast<int>(b->items.size()), std::max(1, cus) * 4);
+  *out = b;
+  return SBLAS_OK;
+}
+
+int sblas_batch_run(sblas_batch b, void *stream) {
+  sblas::clear_error();
+  if (!b) return SBLAS_ERR_ARGS;
+  return b->enqueue(static_cast<hipStream_t>(stream));
+}
+
+int sblas_batch_graph_launch(sblas_batch b, void *stream) {
+  sblas::clear_error();
+  if (!b) return SBLAS_ERR_ARGS;
+  if (!b->graph_exec) {
+    // capture on a private stream (kernel launches only; nothing to clear)
+    SBLAS_HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    SBLAS_HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int st = b->enqueue(b->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (st != SBLAS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) {
+      sblas::set_error(SBLAS_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+      return SBLAS_ERR_HIP;
+    }
+    b->graph = g;
+    SBLAS_HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+  }
+  SBLAS_HIP_TRY(hipGraphLaunch(b->graph_exec, static_cast<hipStream_t>(stream)));
+  return SBLAS_OK;
+}
+
+int sblas_batch_get_choice(sblas_batch b, int index, sblas_batch_choice *out) {
+  if (!b || !out || index < 0 || index >= static_cast<int>(b->choices.size()))
+    return SBLAS_ERR_ARGS;
+  *out = b->choices[index];
+  return SBLAS_OK;
+}
+
+int sblas_batch_get_shape(sblas_batch b, int *work_items, int *grid, int *launches) {
+  if (!b) return SBLAS_ERR_ARGS;
+  if (work_items) *work_items = static_cast<int>(b->items.size());
+  if (grid) *grid = b->grid;
+  if (launches) *launches = (b->items.empty() ? 0 : 1) + (b->groups > 0 ? 1 : 0);
+  return SBLAS_OK;
+}
+
+int sblas_batch_destroy(sblas_batch b) {
+  delete b;
+  return SBLAS_OK;
+}
+
+}  // extern "C"

@@ -91,6 +91,9 @@ void sample_distinct(Rng &g, int64_t lo, int64_t w, int k, int *out) {
 
 extern "C" {
 
+int64_t sblas_gen_rmat_n(int scale, int64_t edges, double a, double b, double c, uint64_t seed,
+                         int64_t n, int **rp_out, int **col_out);
+
 // Row lengths from p(L) ~ (L + shift)^-expo on [lmin, lmax].
 int sblas_gen_rowlen_powerlaw(int m, uint64_t seed, int lmin, int lmax, double shift,
                               double expo, int *rowlen) {
@@ -174,8 +177,17 @@ int sblas_gen_values(int64_t count, uint64_t seed, uint64_t stream_tag, int kind
 // -1 on failure.
 int64_t sblas_gen_rmat(int scale, int64_t edges, double a, double b, double c, uint64_t seed,
                        int **rp_out, int **col_out) {
-  if (scale < 1 || scale > 30 || edges < 0 || !rp_out || !col_out) return -1;
-  const int64_t m = int64_t{1} << scale;
+  return sblas_gen_rmat_n(scale, edges, a, b, c, seed, int64_t{1} << scale, rp_out, col_out);
+}
+
+// As sblas_gen_rmat on the first n vertices: edges with an endpoint >= n are
+// dropped (an R-MAT graph at the next power of two, cut to n).
+int64_t sblas_gen_rmat_n(int scale, int64_t edges, double a, double b, double c, uint64_t seed,
+                         int64_t n, int **rp_out, int **col_out) {
+  if (scale < 1 || scale > 30 || edges < 0 || !rp_out || !col_out || n < 1 ||
+      n > (int64_t{1} << scale))
+    return -1;
+  const int64_t m = n;
   std::vector<int> er(static_cast<size_t>(edges)), ec(static_cast<size_t>(edges));
   const double ab = a + b, abc = a + b + c;
 #pragma omp parallel for schedule(static)
@@ -193,19 +205,22 @@ int64_t sblas_gen_rmat(int scale, int64_t edges, double a, double b, double c, u
       r = (r << 1) | rb;
       cc = (cc << 1) | cb;
     }
-    er[e] = static_cast<int>(r);
+    const bool keep = static_cast<int64_t>(r) < m && static_cast<int64_t>(cc) < m;
+    er[e] = keep ? static_cast<int>(r) : -1;
     ec[e] = static_cast<int>(cc);
   }
   std::vector<std::atomic<int64_t>> cnt(static_cast<size_t>(m));
   for (auto &x : cnt) x.store(0, std::memory_order_relaxed);
 #pragma omp parallel for schedule(static)
-  for (int64_t e = 0; e < edges; e++) cnt[er[e]].fetch_add(1, std::memory_order_relaxed);
+  for (int64_t e = 0; e < edges; e++)
+    if (er[e] >= 0) cnt[er[e]].fetch_add(1, std::memory_order_relaxed);
   std::vector<int64_t> start(static_cast<size_t>(m) + 1, 0);
   for (int64_t r = 0; r < m; r++) start[r + 1] = start[r] + cnt[r].load(std::memory_order_relaxed);
   std::vector<int> tmp(static_cast<size_t>(edges));
   for (int64_t r = 0; r < m; r++) cnt[r].store(start[r], std::memory_order_relaxed);
 #pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < edges; e++) {
+    if (er[e] < 0) continue;
     const int64_t pos = cnt[er[e]].fetch_add(1, std::memory_order_relaxed);
     tmp[pos] = ec[e];
   }

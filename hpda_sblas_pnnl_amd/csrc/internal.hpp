@@ -116,6 +116,31 @@ int launch_coo_tiles(int val_type, int sub, int m, int64_t nnz, int tiles, const
 int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha, double beta,
                       hipStream_t s);
 
+// batched matrices (sblas_batch.hip)
+struct BatchMatrix {
+  const int *rp;
+  const int *col;
+  const void *val;
+  const double *x;
+  double *y;
+  double alpha, beta;
+  int val_type;
+  int subwave;  // lanes per row in the LDS reduction (1..64)
+  int m;
+  int pad_;
+};
+struct BatchItem {  // one row block of one matrix
+  int mat;
+  int row_begin, row_end;
+  int lo, hi;
+  int pad_[3];
+};
+int launch_batch_rowblocks(int items_per_thread, int grid, int nitems, const BatchItem *items,
+                           const BatchMatrix *mats, double *partials, hipStream_t s);
+int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,
+                       const int *gslot, const BatchMatrix *mats, const double *partials,
+                       hipStream_t s);
+
 // CSR5-style tiles (sblas_csr5.hip): 64 lanes x sigma elements per tile
 int launch_csr5(int val_type, int sigma, int ntiles, int64_t nnz, const uint32_t *tile_nz,
                 const uint32_t *flags, const int *rows_nz, const int *colT, const void *valT,

@@ -58,6 +58,63 @@ def _stream_ptr(stream):
     return ctypes.c_void_p(s.cuda_stream)
 
 
+class SpmvBatch:
+    """Many CSR matrices run as one persistent launch (optionally a replayed
+    hipGraph).  items: list of dicts with keys m, n, row_ptr, col, val, x, y,
+    alpha, beta (tensors on one HIP device)."""
+
+    def __init__(self, items, subwave=0, chunks=0, item_size=0):
+        self._keep = []
+        arr = (_lib.BatchItem * max(len(items), 1))()
+        for i, it in enumerate(items):
+            self._keep.append(it)
+            arr[i].m, arr[i].n = int(it["m"]), int(it["n"])
+            arr[i].nnz = int(it["val"].numel())
+            arr[i].d_row_ptr = it["row_ptr"].data_ptr()
+            arr[i].d_col = it["col"].data_ptr()
+            arr[i].d_val = it["val"].data_ptr()
+            arr[i].val_type = val_type_of(it["val"])
+            arr[i].d_x = it["x"].data_ptr()
+            arr[i].d_y = it["y"].data_ptr()
+            arr[i].alpha = float(it.get("alpha", 1.0))
+            arr[i].beta = float(it.get("beta", 0.0))
+        o = make_opts(subwave=subwave, chunks=chunks, items=item_size)
+        h = ctypes.c_void_p()
+        check(lib().sblas_batch_create(len(items), arr, ctypes.byref(o), ctypes.byref(h)),
+              "sblas_batch_create")
+        self._h = h
+        self.count = len(items)
+
+    def run(self, stream=None):
+        check(lib().sblas_batch_run(self._h, _stream_ptr(stream)), "sblas_batch_run")
+
+    def graph_launch(self, stream=None):
+        check(lib().sblas_batch_graph_launch(self._h, _stream_ptr(stream)),
+              "sblas_batch_graph_launch")
+
+    def choice(self, i) -> dict:
+        c = _lib.BatchChoice()
+        check(lib().sblas_batch_get_choice(self._h, i, ctypes.byref(c)), "sblas_batch_get_choice")
+        return c.as_dict()
+
+    def shape(self) -> dict:
+        w, g, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().sblas_batch_get_shape(self._h, ctypes.byref(w), ctypes.byref(g),
+                                          ctypes.byref(n)), "sblas_batch_get_shape")
+        return {"work_items": w.value, "grid": g.value, "launches": n.value}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().sblas_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001  (interpreter shutdown)
+            pass
+
+
 class SpmvPlan:
     """A matrix analysed for repeated y = alpha*A*x + beta*y.
 

@@ -85,12 +85,17 @@ def banded_csr(n: int = 2_000_000, seed: int = 0, mu: float = 140.0, sigma: floa
 
 
 def rmat_csr(scale: int = 22, edge_factor: int = 17, a: float = 0.57, b: float = 0.19,
-             c: float = 0.19, seed: int = 0) -> CSR:
-    m = 1 << scale
+             c: float = 0.19, seed: int = 0, n: int | None = None) -> CSR:
+    """R-MAT with 2^scale vertices; with n, the graph drawn at the next power of
+    two >= n and cut to its first n vertices (edges with an endpoint >= n are
+    dropped)."""
+    if n is not None:
+        scale = max(1, int(n - 1).bit_length())
+    m = (1 << scale) if n is None else int(n)
     rpp = ctypes.POINTER(ctypes.c_int)()
     cpp = ctypes.POINTER(ctypes.c_int)()
-    nnz = gen().sblas_gen_rmat(scale, edge_factor * m, a, b, c, seed, ctypes.byref(rpp),
-                               ctypes.byref(cpp))
+    nnz = gen().sblas_gen_rmat_n(scale, edge_factor * (1 << scale), a, b, c, seed, m,
+                                 ctypes.byref(rpp), ctypes.byref(cpp))
     if nnz < 0:
         raise RuntimeError("R-MAT generation failed")
     try:
@@ -100,6 +105,23 @@ def rmat_csr(scale: int = 22, edge_factor: int = 17, a: float = 0.57, b: float =
         gen().sblas_gen_free(ctypes.cast(rpp, ctypes.c_void_p))
         gen().sblas_gen_free(ctypes.cast(cpp, ctypes.c_void_p))
     return CSR(m, m, rp, col, np.ones(nnz, dtype=np.float64))
+
+
+def batched_workload(count: int = 64, seed: int = 0, n_lo: int = 10_000, n_hi: int = 2_000_000):
+    """count matrices with n log-uniform in [n_lo, n_hi] (seeded): even index a
+    banded matrix (rows N(140, 40) clipped to [8, 400], half-width
+    min(50,000, n/40)), odd index R-MAT (edge factor 16) drawn at the next power
+    of two and cut to n.  Returns [(kind, CSR)]."""
+    u = values(count, seed, 77, "uniform") * 0.5 + 0.5
+    ns = np.rint(np.exp(np.log(n_lo) + u * (np.log(n_hi) - np.log(n_lo)))).astype(np.int64)
+    out = []
+    for i, n in enumerate(ns):
+        n = int(n)
+        if i % 2 == 0:
+            out.append(("banded", banded_csr(n, seed=seed + i, halfwidth=min(50_000, n // 40))))
+        else:
+            out.append(("rmat", rmat_csr(edge_factor=16, seed=seed + i, n=n)))
+    return out
 
 
 def to_csc(A: CSR):

@@ -244,6 +244,49 @@ int sblas_plan_destroy(sblas_plan plan);
 int sblas_plan_set_timing(sblas_plan plan, int on);
 int sblas_plan_kernel_times(sblas_plan plan, float *ms, int cap);
 
+/* ---------------------------------------------------------------------- */
+/* batched matrices: one persistent launch, replayable as a hipGraph        */
+/* ---------------------------------------------------------------------- */
+/* No counterpart in the reference (its calls take one matrix).  Each item is
+ * a CSR matrix on the device with its own x, y, alpha, beta; arrays must stay
+ * alive and unchanged while the batch exists. */
+typedef struct sblas_batch_item {
+  int m, n;
+  int64_t nnz;
+  const int *d_row_ptr;
+  const int *d_col;
+  const void *d_val;
+  int val_type;
+  int pad_;
+  const double *d_x;
+  double *d_y;
+  double alpha, beta;
+} sblas_batch_item;
+
+/* What the autotuner picked for one matrix. */
+typedef struct sblas_batch_choice {
+  int kernel;      /* SBLAS_KERNEL_VECTOR (row-block form)                  */
+  int subwave;     /* lanes per row                                          */
+  int items;       /* elements per thread of a row block                     */
+  int chunks;      /* nnz chunks the row blocks are cut from                 */
+  int row_blocks;  /* work items of this matrix                              */
+  int pad_;
+  double mean_row, cv_row;
+  int64_t max_row;
+} sblas_batch_choice;
+
+typedef struct sblas_batch_s *sblas_batch;
+int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opts *opts,
+                       sblas_batch *batch);
+/* y_i = alpha_i*A_i*x_i + beta_i*y_i for every item, launched directly. */
+int sblas_batch_run(sblas_batch batch, void *stream);
+/* The same work as one hipGraph (captured on first use, replayed after). */
+int sblas_batch_graph_launch(sblas_batch batch, void *stream);
+int sblas_batch_get_choice(sblas_batch batch, int index, sblas_batch_choice *out);
+/* work items, persistent grid size, kernel launches per run */
+int sblas_batch_get_shape(sblas_batch batch, int *work_items, int *grid, int *launches);
+int sblas_batch_destroy(sblas_batch batch);
+
 #ifdef __cplusplus
 }
 #endif
