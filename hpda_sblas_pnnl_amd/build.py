@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "sblas_csc.hip",
     "sblas_csr5.hip",
     "sblas_batch.hip",
+    "sblas_panel.hip",
     "batch.cpp",
     "plan.cpp",
     "dropin.cpp",

@@ -116,6 +116,32 @@ int launch_coo_tiles(int val_type, int sub, int m, int64_t nnz, int tiles, const
 int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha, double beta,
                       hipStream_t s);
 
+// column-panel form of kernel 1 (sblas_panel.hip)
+constexpr int kPanelCols = 4096;  // x panel staged in LDS (32 KB of FP64)
+struct PanelFormat {
+  int rpt = 4;        // rows per thread: row panel of 256*rpt rows
+  int npanels = 0, nslices = 0;
+  double avg_panels_per_row_panel = 0, avg_slice = 0;
+  int *q_lo = nullptr;          // first column panel of each row panel
+  int *slice_base = nullptr;    // first slice of each row panel (npanels+1)
+  int64_t *slice_off = nullptr; // element offset of each slice (nslices+1)
+  // per slice: R+1 row offsets inside the slice, 16-bit when every slice
+  // holds at most 65535 elements, else 32-bit (srp32)
+  uint16_t *srp = nullptr;
+  uint32_t *srp32 = nullptr;
+  void *pval = nullptr;         // values, slice-major
+  uint16_t *pcol = nullptr;     // column offsets inside the column panel
+  std::vector<void *> buffers;
+};
+// Builds the format on the device.  With force == false it stops after the
+// per-panel column ranges and sets *chosen = false when the form would not
+// help (few row panels, sparse slices, or rows whose x window is narrow).
+int panel_build(int val_type, int rpt, int m, int n, int64_t nnz, const int *rp, const int *col,
+                const void *val, bool force, int min_panels, PanelFormat &f, bool *chosen,
+                hipStream_t s);
+int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
+                      const double *x, double *y, double alpha, double beta, hipStream_t s);
+
 // batched matrices (sblas_batch.hip)
 struct BatchMatrix {
   const int *rp;

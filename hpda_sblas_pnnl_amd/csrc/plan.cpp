@@ -97,6 +97,7 @@ int autotune_sigma(const RowStats &s) {
 // ---------------------------------------------------------------------------
 
 Plan::~Plan() {
+  for (void *p : panel.buffers) (void)hipFree(p);
   for (hipEvent_t e : tev) (void)hipEventDestroy(e);
   for (void *p : device_buffers) (void)hipFree(p);
   for (hipStream_t s : streams) (void)hipStreamDestroy(s);
@@ -410,6 +411,15 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     set_error(SBLAS_ERR_UNSUPPORTED, "multi-vector products need a CSR or CSC plan");
     return SBLAS_ERR_UNSUPPORTED;
   }
+  if (variant == 3 && k == 1 && streams.empty()) {
+    hipEvent_t *te = timing_slot();
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
+    st = launch_panel_spmv(val_type, panel, m, n, vl.xcd_remap, X, Y, alpha, beta, s);
+    if (st != SBLAS_OK) return st;
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
+    launches++;
+    return SBLAS_OK;
+  }
   if (variant == 2 && k == 1 && streams.empty()) {
     hipEvent_t *te = timing_slot();
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
@@ -552,6 +562,33 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
   if ((st = setup_vector(h_rp.data())) != SBLAS_OK) return st;
   if (kernel == SBLAS_KERNEL_MERGE) return setup_tiles();
   if (kernel == SBLAS_KERNEL_CSR5) return setup_csr5(h_rp);
+  if (variant == 2 && (opts.variant == 0 || opts.variant == 3) && opts.streams <= 1 && nnz > 0) {
+    // column-panel form: forced by variant 3, else taken when it pays
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    const int min_panels = 2 * std::max(cus, 1);
+    const int rpt = (m / 1024 >= min_panels) ? 4 : 2;
+    bool chosen = false;
+    st = panel_build(val_type, rpt, m, n, nnz, a0, a1, val, opts.variant == 3, min_panels, panel,
+                     &chosen, nullptr);
+    if (st == SBLAS_ERR_UNSUPPORTED && opts.variant == 0) {
+      // a slice too large for 16-bit offsets: keep the row-block form
+      for (void *p : panel.buffers) (void)hipFree(p);
+      panel.buffers.clear();
+      panel.npanels = 0;
+      clear_error();
+      chosen = false;
+    } else if (st != SBLAS_OK) {
+      return st;
+    }
+    if (chosen) {
+      variant = 3;
+      int remap = opts.xcd_remap;
+      if (remap < 0) remap = env_int("SBLAS_XCD_REMAP", 1);
+      vl.xcd_remap = remap ? 1 : 0;
+      return SBLAS_OK;
+    }
+  }
   if (variant == 2) return setup_stream(h_rp.data());
   return SBLAS_OK;
 }
@@ -563,12 +600,14 @@ bool Plan::uses_tiles(int k) const {
 
 bool Plan::has_merge(int k) const {
   if (uses_tiles(k)) return tiles > 0;
+  if (variant == 3 && k == 1 && streams.empty()) return false;  // rows never cut
   if (variant == 2 && k == 1 && streams.empty()) return bgroups > 0;
   return ngroups > 0;
 }
 
 int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) {
   if (uses_tiles(k)) return tiles > 0 ? launch_tile_fixup(tiles, d_carry, Y, alpha, beta, s) : SBLAS_OK;
+  if (variant == 3 && k == 1 && streams.empty()) return SBLAS_OK;
   if (variant == 2 && k == 1 && streams.empty())
     return bgroups > 0 ? launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, 1, Y,
                                       alpha, beta, s)

@@ -69,6 +69,9 @@ struct Plan {
   int *d_tile_row = nullptr;
   uint32_t *d_tile_flags = nullptr;
 
+  // column-panel form of kernel 1 (variant 3)
+  PanelFormat panel;
+
   // CSR5-style format (kernel 3)
   uint32_t *d_c5_tile_nz = nullptr, *d_c5_flags = nullptr;
   int *d_c5_rows_nz = nullptr, *d_c5_empty = nullptr;

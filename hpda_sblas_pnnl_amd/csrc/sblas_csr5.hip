@@ -108,7 +108,9 @@ __global__ __launch_bounds__(256) void k_csr5(
 
   // segmented inclusive scan of a; a segment begins after each lane with a start
   double sv = a;
-  int head = (lane == 0) ? 1 : (__shfl_up(has_start ? 1 : 0, 1, 64) != 0);
+  // every lane takes part in each shuffle; the select comes after
+  const int prev_has = __shfl_up(has_start ? 1 : 0, 1, 64);
+  int head = (lane == 0) ? 1 : prev_has;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const double vu = __shfl_up(sv, d, 64);
