@@ -63,7 +63,7 @@ assert CHUNK_DTYPE.itemsize == ctypes.sizeof(ChunkDesc) == 48
 class Opts(ctypes.Structure):
     _fields_ = [("kernel", _I), ("chunks", _I), ("part_mode", _I), ("nb", _I),
                 ("streams", _I), ("subwave", _I), ("sigma", _I), ("xcd_remap", _I),
-                ("variant", _I), ("items", _I), ("reserved", _I * 6)]
+                ("variant", _I), ("items", _I), ("panel_rows", _I), ("reserved", _I * 5)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -71,7 +71,8 @@ class PlanInfo(ctypes.Structure):
                 ("subwave", _I), ("sigma", _I), ("xcd_remap", _I), ("split_rows", _I),
                 ("m", _I64), ("n", _I64), ("nnz", _I64), ("workspace_bytes", _I64),
                 ("mean_row", _D), ("cv_row", _D), ("max_row", _I64), ("launches", _I),
-                ("variant", _I), ("row_blocks", _I), ("items", _I), ("reserved", _I * 4)]
+                ("variant", _I), ("row_blocks", _I), ("items", _I),
+                ("panel_rows", _I), ("panel_slices", _I), ("reserved", _I * 2)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -72,6 +72,12 @@ struct StreamLaunch {
 int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *blocks,
                       const int *rp, const int *col, const void *val, const double *x,
                       double *y, double alpha, double beta, double *partials, hipStream_t s);
+// k in {2, 4, 8, 16} right-hand sides over the same row blocks; partials hold
+// 2 slots x k per block.
+int launch_csr_blocks_mrhs(int val_type, int k, const StreamLaunch &L, const RowBlock *blocks,
+                           const int *rp, const int *col, const void *val, const double *X,
+                           double *Y, double alpha, double beta, double *partials,
+                           hipStream_t s);
 
 // ---- launchers (sblas_kernels.hip) --------------------------------------------
 struct VectorLaunch {
@@ -118,8 +124,10 @@ int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha
 
 // column-panel form of kernel 1 (sblas_panel.hip)
 constexpr int kPanelCols = 4096;  // x panel staged in LDS (32 KB of FP64)
+constexpr int kPanelRowsMax = 2048;  // rows per row panel (8 per thread)
 struct PanelFormat {
-  int rpt = 4;        // rows per thread: row panel of 256*rpt rows
+  int rows = 0;       // rows per row panel
+  int rpt = 1;        // rows per thread: ceil(rows/256) rounded to 1, 2, 4, 6 or 8
   int npanels = 0, nslices = 0;
   double avg_panels_per_row_panel = 0, avg_slice = 0;
   int *q_lo = nullptr;          // first column panel of each row panel
@@ -133,10 +141,14 @@ struct PanelFormat {
   uint16_t *pcol = nullptr;     // column offsets inside the column panel
   std::vector<void *> buffers;
 };
-// Builds the format on the device.  With force == false it stops after the
-// per-panel column ranges and sets *chosen = false when the form would not
-// help (few row panels, sparse slices, or rows whose x window is narrow).
-int panel_build(int val_type, int rpt, int m, int n, int64_t nnz, const int *rp, const int *col,
+int panel_rpt(int rows);
+// Rows per row panel so the panels fill whole rounds of resident workgroups.
+int panel_rows_for(int val_type, int m, int device);
+// Builds the format on the device with row panels of `rows` rows.  With
+// force == false it stops after the per-panel column ranges and sets
+// *chosen = false when the form would not help (few row panels, sparse
+// slices, or rows whose x window is narrow).
+int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp, const int *col,
                 const void *val, bool force, int min_panels, PanelFormat &f, bool *chosen,
                 hipStream_t s);
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,

@@ -347,7 +347,9 @@ int Plan::setup_stream(const int *h_rp) {
   if ((st = upload(*this, &d_bmrows, mrows)) != SBLAS_OK) return st;
   if ((st = upload(*this, &d_bmptr, mptr)) != SBLAS_OK) return st;
   if ((st = upload(*this, &d_bmslots, mslots)) != SBLAS_OK) return st;
-  d_bpartials = static_cast<double *>(dev_alloc(sizeof(double) * 2 * std::max<size_t>(rblocks.size(), 1)));
+  // 2 slots per block, max_k values per slot (multi-RHS blocks use the same list)
+  d_bpartials = static_cast<double *>(
+      dev_alloc(sizeof(double) * 2 * max_k * std::max<size_t>(rblocks.size(), 1)));
   if (!d_bpartials) {
     set_error(SBLAS_ERR_ALLOC, "plan: partials allocation failed");
     return SBLAS_ERR_ALLOC;
@@ -430,6 +432,21 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     launches++;
     if (bgroups > 0 && merge_enabled) {
       st = launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, 1, Y, alpha, beta, s);
+      if (st != SBLAS_OK) return st;
+      launches++;
+    }
+    return SBLAS_OK;
+  }
+  if (variant >= 2 && k > 1 && streams.empty()) {
+    hipEvent_t *te = timing_slot();
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
+    st = launch_csr_blocks_mrhs(val_type, k, sl, d_rblocks, a0, a1, val, X, Y, alpha, beta,
+                                d_bpartials, s);
+    if (st != SBLAS_OK) return st;
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
+    launches++;
+    if (bgroups > 0 && merge_enabled) {
+      st = launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, k, Y, alpha, beta, s);
       if (st != SBLAS_OK) return st;
       launches++;
     }
@@ -567,9 +584,9 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     const int min_panels = 2 * std::max(cus, 1);
-    const int rpt = (m / 1024 >= min_panels) ? 4 : 2;
+    const int rows = opts.panel_rows > 0 ? opts.panel_rows : panel_rows_for(val_type, m, device);
     bool chosen = false;
-    st = panel_build(val_type, rpt, m, n, nnz, a0, a1, val, opts.variant == 3, min_panels, panel,
+    st = panel_build(val_type, rows, m, n, nnz, a0, a1, val, opts.variant == 3, min_panels, panel,
                      &chosen, nullptr);
     if (st == SBLAS_ERR_UNSUPPORTED && opts.variant == 0) {
       // a slice too large for 16-bit offsets: keep the row-block form
@@ -586,7 +603,7 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
       int remap = opts.xcd_remap;
       if (remap < 0) remap = env_int("SBLAS_XCD_REMAP", 1);
       vl.xcd_remap = remap ? 1 : 0;
-      return SBLAS_OK;
+      return setup_stream(h_rp.data());  // row blocks serve multi-RHS products
     }
   }
   if (variant == 2) return setup_stream(h_rp.data());
@@ -601,15 +618,15 @@ bool Plan::uses_tiles(int k) const {
 bool Plan::has_merge(int k) const {
   if (uses_tiles(k)) return tiles > 0;
   if (variant == 3 && k == 1 && streams.empty()) return false;  // rows never cut
-  if (variant == 2 && k == 1 && streams.empty()) return bgroups > 0;
+  if (variant >= 2 && streams.empty()) return bgroups > 0;  // row blocks (any k)
   return ngroups > 0;
 }
 
 int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) {
   if (uses_tiles(k)) return tiles > 0 ? launch_tile_fixup(tiles, d_carry, Y, alpha, beta, s) : SBLAS_OK;
   if (variant == 3 && k == 1 && streams.empty()) return SBLAS_OK;
-  if (variant == 2 && k == 1 && streams.empty())
-    return bgroups > 0 ? launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, 1, Y,
+  if (variant >= 2 && streams.empty())
+    return bgroups > 0 ? launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, k, Y,
                                       alpha, beta, s)
                        : SBLAS_OK;
   return ngroups > 0
@@ -887,6 +904,8 @@ int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info) {
   info->variant = p.variant;
   info->row_blocks = static_cast<int>(p.rblocks.size());
   info->items = p.sl.items;
+  info->panel_rows = p.variant == 3 ? p.panel.rows : 0;
+  info->panel_slices = p.variant == 3 ? p.panel.nslices : 0;
   if (p.variant == 2) info->subwave = p.sl.subwave;
   return SBLAS_OK;
 }

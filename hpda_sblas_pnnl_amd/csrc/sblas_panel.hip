@@ -7,16 +7,17 @@
 // into row panels of R rows and column panels of W columns.  A workgroup owns
 // one row panel and walks its column panels in increasing order: it stages
 // x[q*W, (q+1)*W) in LDS with coalesced loads (one pass over the panel's x
-// lines), then every thread adds the elements of its rows that fall in that
-// column panel, reading x from LDS.  The plan stores each (row panel, column
-// panel) slice contiguously, rows in order, with 16-bit column offsets inside
-// the panel and row offsets inside the slice (16-bit when every slice holds
-// at most 65535 elements, else 32-bit).
+// lines), then streams that slice's elements in chunks of 2048 with
+// consecutive threads on consecutive elements, multiplies each by its x from
+// LDS and leaves the products in LDS.  The plan stores each (row panel,
+// column panel) slice contiguously, rows in order, with 16-bit column offsets
+// inside the panel and row offsets inside the slice (16-bit when every slice
+// holds at most 65535 elements, else 32-bit).
 //
-// Each row is summed by one thread, element by element in ascending column
-// order, with separately rounded multiplies and adds: the same operations in
-// the same order as the reference's host loop (test/dspmv_test_smt.cu:45-51),
-// so results equal the host oracle bit for bit.
+// Each row is then summed by one thread from the staged products, element by
+// element in ascending column order, with separately rounded multiplies and
+// adds: the same operations in the same order as the reference's host loop
+// (test/dspmv_test_smt.cu:45-51), so results equal the host oracle bit for bit.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -43,16 +44,49 @@ __device__ __forceinline__ double pv(const uint16_t *p, int64_t j) {
   return static_cast<double>(__uint_as_float(h << 16));
 }
 
+// Separately rounded multiply and add.  Contraction is switched off inside
+// these bodies, so a product never fuses with the add that consumes it (the
+// header intrinsics are plain * and + and would fuse under -ffp-contract=fast).
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
+// Slice s of the current row panel and the first element of the chunk being
+// worked on; advance() moves to the next chunk, skipping empty slices.
+struct PanelCursor {
+  int s, k0, len;
+  int64_t base;
+};
+
+__device__ __forceinline__ void cursor_seek(PanelCursor &c, int s, int s1,
+                                            const int64_t *__restrict__ slice_off) {
+  c.k0 = 0;
+  c.s = s;
+  c.len = 0;
+  for (; c.s < s1; c.s++) {
+    c.base = slice_off[c.s];
+    c.len = static_cast<int>(slice_off[c.s + 1] - c.base);
+    if (c.len > 0) return;
+  }
+}
+
 template <typename VT, int RPT, typename OT>
 __global__ __launch_bounds__(256) void k_panel_spmv(
-    int m, int n, int npanels, int xcd_remap, const int *__restrict__ q_lo,
+    int m, int n, int R, int npanels, int xcd_remap, const int *__restrict__ q_lo,
     const int *__restrict__ slice_base, const int64_t *__restrict__ slice_off,
     const OT *__restrict__ srp, const VT *__restrict__ pval,
     const uint16_t *__restrict__ pcol, const double *__restrict__ x, double *__restrict__ y,
     double alpha, double beta) {
-  constexpr int R = 256 * RPT;
   constexpr int W = kPanelCols;
+  constexpr int ITEMS = 8;
+  constexpr int CH = 256 * ITEMS;  // elements per chunk of a slice
   __shared__ double xs[W];
+  __shared__ double prod[CH];
   int p = blockIdx.x;
   if (xcd_remap) {
     const int per = gridDim.x >> 3;
@@ -61,52 +95,87 @@ __global__ __launch_bounds__(256) void k_panel_spmv(
   if (p >= npanels) return;  // uniform per workgroup
   const int t = threadIdx.x;
   const int r0 = p * R;
-  double acc[RPT];
-#pragma unroll
-  for (int j = 0; j < RPT; j++) acc[j] = 0.0;
   const int s0 = slice_base[p], s1 = slice_base[p + 1];
   const int qb = q_lo[p];
-  for (int s = s0; s < s1; s++) {
+  double acc[RPT];
+  int b[RPT], e[RPT];
+  double v[ITEMS];
+  int l[ITEMS];
+#pragma unroll
+  for (int j = 0; j < RPT; j++) acc[j] = 0.0;
+
+  // x[q*W, (q+1)*W) of slice s into LDS
+  auto stage_x = [&](int s) {
     const int c0 = (qb + (s - s0)) * W;
-    __syncthreads();  // previous panel of x no longer read
 #pragma unroll
     for (int i = 0; i < W / 256; i++) {
       const int c = c0 + t + i * 256;
       const double xv = x[min(c, n - 1)];  // no load under a branch
       xs[t + i * 256] = (c < n) ? xv : 0.0;
     }
-    __syncthreads();
-    const int64_t base = slice_off[s];
+  };
+  // element ranges [b, e) of this thread's rows inside slice s
+  auto row_ranges = [&](int s) {
     const OT *rps = srp + static_cast<int64_t>(s) * (R + 1);
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       const int rl = t + 256 * j;
-      const int b = rps[rl], e = rps[rl + 1];
+      const bool in = rl < R;
+      const int lo = rps[in ? rl : 0], hi = rps[in ? rl + 1 : 0];  // unconditional loads
+      b[j] = in ? lo : 0;
+      e[j] = in ? hi : 0;
+    }
+  };
+  // values and column offsets of chunk c: consecutive threads on consecutive
+  // elements (whole lines per load instruction), all issued back to back
+  auto load_chunk = [&](const PanelCursor &c) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+      const int64_t k = c.base + min(c.k0 + t + i * 256, c.len - 1);
+      v[i] = pv(pval, k);
+      l[i] = __builtin_nontemporal_load(pcol + k);
+    }
+  };
+
+  PanelCursor cur;
+  cursor_seek(cur, s0, s1, slice_off);
+  if (cur.s < s1) {
+    stage_x(cur.s);
+    row_ranges(cur.s);
+    load_chunk(cur);
+  }
+  __syncthreads();
+  while (cur.s < s1) {
+    // products of the current chunk (x of its slice is in LDS)
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) prod[t + i * 256] = mul_rn(v[i], xs[l[i]]);
+    PanelCursor nxt = cur;
+    nxt.k0 += CH;
+    if (nxt.k0 >= cur.len) cursor_seek(nxt, cur.s + 1, s1, slice_off);
+    // the next chunk's loads are in flight during the row sums below
+    if (nxt.s < s1) load_chunk(nxt);
+    __syncthreads();  // products visible; x of the current slice no longer read
+    const bool new_slice = nxt.s != cur.s && nxt.s < s1;
+    if (new_slice) stage_x(nxt.s);
+    // each row adds its products of this chunk in element order
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      const int lo = max(b[j], cur.k0), hi = min(e[j], cur.k0 + CH);
       double a = acc[j];
-      int k = b;
-      // four elements' loads in flight, then the ordered adds
-      for (; k + 4 <= e; k += 4) {
-        const double v0 = pv(pval, base + k), v1 = pv(pval, base + k + 1);
-        const double v2 = pv(pval, base + k + 2), v3 = pv(pval, base + k + 3);
-        const int l0 = __builtin_nontemporal_load(pcol + base + k);
-        const int l1 = __builtin_nontemporal_load(pcol + base + k + 1);
-        const int l2 = __builtin_nontemporal_load(pcol + base + k + 2);
-        const int l3 = __builtin_nontemporal_load(pcol + base + k + 3);
-        a = __dadd_rn(a, __dmul_rn(v0, xs[l0]));
-        a = __dadd_rn(a, __dmul_rn(v1, xs[l1]));
-        a = __dadd_rn(a, __dmul_rn(v2, xs[l2]));
-        a = __dadd_rn(a, __dmul_rn(v3, xs[l3]));
-      }
-      for (; k < e; k++) a = __dadd_rn(a, __dmul_rn(pv(pval, base + k), xs[pcol[base + k]]));
+      for (int k = lo; k < hi; k++) a = add_rn(a, prod[k - cur.k0]);
       acc[j] = a;
     }
+    if (new_slice) row_ranges(nxt.s);
+    __syncthreads();  // products consumed; x of the next slice visible
+    cur = nxt;
   }
 #pragma unroll
   for (int j = 0; j < RPT; j++) {
-    const int r = r0 + t + 256 * j;
-    if (r < m) {
-      y[r] = (beta == 0.0) ? __dmul_rn(alpha, acc[j])
-                           : __dadd_rn(__dmul_rn(alpha, acc[j]), __dmul_rn(beta, y[r]));
+    const int rl = t + 256 * j;
+    const int r = r0 + rl;
+    if (rl < R && r < m) {
+      y[r] = (beta == 0.0) ? mul_rn(alpha, acc[j])
+                           : add_rn(mul_rn(alpha, acc[j]), mul_rn(beta, y[r]));
     }
   }
 }
@@ -172,13 +241,13 @@ __global__ void k_panel_count(int m, int R, const int *__restrict__ rp,
   if (cur >= 0) cnt[static_cast<int64_t>(sb + cur - qb) * R + rl] = c;
 }
 
-// Exclusive scan of each slice's R counts -> 16-bit row offsets (R+1 per
-// slice) and the slice's element count.  One workgroup per slice.
+// Exclusive scan of each slice's R counts -> row offsets (R+1 per slice) and
+// the slice's element count; flags slices above 65535 elements.  One
+// workgroup per slice; thread t scans rows t*RPT .. t*RPT+RPT-1 (R <= 256*RPT).
 template <int RPT>
-__global__ void k_panel_slice_scan(int nslices, const uint32_t *__restrict__ cnt,
+__global__ void k_panel_slice_scan(int nslices, int R, const uint32_t *__restrict__ cnt,
                                    uint32_t *__restrict__ srp, int64_t *__restrict__ snnz,
                                    int *__restrict__ overflow) {
-  constexpr int R = 256 * RPT;
   const int s = blockIdx.x;
   if (s >= nslices) return;
   typedef hipcub::BlockScan<uint32_t, 256> Scan;
@@ -187,7 +256,8 @@ __global__ void k_panel_slice_scan(int nslices, const uint32_t *__restrict__ cnt
   uint32_t local = 0;
 #pragma unroll
   for (int j = 0; j < RPT; j++) {
-    v[j] = cnt[static_cast<int64_t>(s) * R + threadIdx.x * RPT + j];
+    const int rl = threadIdx.x * RPT + j;
+    v[j] = rl < R ? cnt[static_cast<int64_t>(s) * R + rl] : 0u;
     local += v[j];
   }
   uint32_t excl = 0, total = 0;
@@ -195,7 +265,8 @@ __global__ void k_panel_slice_scan(int nslices, const uint32_t *__restrict__ cnt
   uint32_t run = excl;
 #pragma unroll
   for (int j = 0; j < RPT; j++) {
-    srp[static_cast<int64_t>(s) * (R + 1) + threadIdx.x * RPT + j] = run;
+    const int rl = threadIdx.x * RPT + j;
+    if (rl < R) srp[static_cast<int64_t>(s) * (R + 1) + rl] = run;
     run += v[j];
   }
   if (threadIdx.x == 0) {
@@ -240,13 +311,18 @@ __global__ void k_panel_scatter(int m, int R, const int *__restrict__ rp,
 
 }  // namespace
 
-int panel_build(int val_type, int rpt, int m, int n, int64_t nnz, const int *rp, const int *col,
+int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp, const int *col,
                 const void *val, bool force, int min_panels, PanelFormat &f, bool *chosen,
                 hipStream_t s) {
   (void)n;
   *chosen = false;
-  const int R = 256 * rpt;
-  f.rpt = rpt;
+  if (rows < 1 || rows > kPanelRowsMax) {
+    set_error(SBLAS_ERR_ARGS, "panel format: rows per panel must be in [1, 2048]");
+    return SBLAS_ERR_ARGS;
+  }
+  const int R = rows;
+  f.rows = R;
+  f.rpt = panel_rpt(R);
   f.npanels = (m + R - 1) / R;
   f.buffers.clear();
   auto alloc = [&](size_t bytes) -> void * {
@@ -321,12 +397,19 @@ int panel_build(int val_type, int rpt, int m, int n, int64_t nnz, const int *rp,
     hipLaunchKernelGGL(k_panel_count, dim3((m + 255) / 256), dim3(256), 0, s, m, R, rp, col,
                        f.q_lo, f.slice_base, cnt);
   if (f.nslices > 0) {
-    if (rpt == 4)
-      hipLaunchKernelGGL(k_panel_slice_scan<4>, dim3(f.nslices), dim3(256), 0, s, f.nslices, cnt,
-                         f.srp32, snnz, overflow);
-    else
-      hipLaunchKernelGGL(k_panel_slice_scan<2>, dim3(f.nslices), dim3(256), 0, s, f.nslices, cnt,
-                         f.srp32, snnz, overflow);
+#define SBLAS_SCAN_CASE(RPT)                                                                  \
+  case RPT:                                                                                   \
+    hipLaunchKernelGGL(k_panel_slice_scan<RPT>, dim3(f.nslices), dim3(256), 0, s, f.nslices, R, \
+                       cnt, f.srp32, snnz, overflow);                                         \
+    break;
+    switch (f.rpt) {
+      SBLAS_SCAN_CASE(1)
+      SBLAS_SCAN_CASE(2)
+      SBLAS_SCAN_CASE(4)
+      SBLAS_SCAN_CASE(6)
+      SBLAS_SCAN_CASE(8)
+    }
+#undef SBLAS_SCAN_CASE
   }
   int h_over = 0;
   SBLAS_HIP_TRY(hipMemcpyAsync(&h_over, overflow, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -392,33 +475,76 @@ int panel_build(int val_type, int rpt, int m, int n, int64_t nnz, const int *rp,
   return SBLAS_OK;
 }
 
+namespace {
+
+template <typename VT, typename OT>
+int panel_launch_typed(const PanelFormat &f, const OT *srp, int m, int n, int xcd_remap,
+                       const double *x, double *y, double alpha, double beta, hipStream_t s) {
+  const int grid = xcd_remap ? ((f.npanels + 7) / 8) * 8 : f.npanels;
+#define SBLAS_PANEL_CASE(RPT)                                                                 \
+  case RPT:                                                                                   \
+    hipLaunchKernelGGL((k_panel_spmv<VT, RPT, OT>), dim3(grid), dim3(256), 0, s, m, n, f.rows, \
+                       f.npanels, xcd_remap, f.q_lo, f.slice_base, f.slice_off, srp,          \
+                       static_cast<const VT *>(f.pval), f.pcol, x, y, alpha, beta);          \
+    break;
+  switch (f.rpt) {
+    SBLAS_PANEL_CASE(1)
+    SBLAS_PANEL_CASE(2)
+    SBLAS_PANEL_CASE(4)
+    SBLAS_PANEL_CASE(6)
+    SBLAS_PANEL_CASE(8)
+    default:
+      set_error(SBLAS_ERR_ARGS, "panel kernel: bad rows per thread");
+      return SBLAS_ERR_ARGS;
+  }
+#undef SBLAS_PANEL_CASE
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+template <typename VT>
+int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const double *x, double *y,
+                    double alpha, double beta, hipStream_t s) {
+  return f.srp ? panel_launch_typed<VT>(f, f.srp, m, n, xcd_remap, x, y, alpha, beta, s)
+               : panel_launch_typed<VT>(f, f.srp32, m, n, xcd_remap, x, y, alpha, beta, s);
+}
+
+}  // namespace
+
+int panel_rpt(int rows) {
+  const int need = (rows + 255) / 256;
+  return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : 8;
+}
+
+int panel_rows_for(int val_type, int m, int device) {
+  int cus = 256, per_cu = 0;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  const void *fn = val_type == SBLAS_VAL_F64
+                       ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t>)
+                   : val_type == SBLAS_VAL_F32
+                       ? reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t>)
+                       : reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 2;
+  const int64_t slots = static_cast<int64_t>(std::max(cus, 1)) * per_cu;
+  // whole rounds of resident workgroups (at least two, so a slow workgroup is
+  // not the whole tail), the tallest panels that fit
+  for (int64_t k = 2;; k++) {
+    int64_t R = (m + k * slots - 1) / (k * slots);
+    R = (R + 63) / 64 * 64;
+    if (R <= kPanelRowsMax) return static_cast<int>(std::max<int64_t>(R, 64));
+  }
+}
+
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
                       const double *x, double *y, double alpha, double beta, hipStream_t s) {
   if (f.npanels <= 0) return SBLAS_OK;
-  const int grid = xcd_remap ? ((f.npanels + 7) / 8) * 8 : f.npanels;
-#define SBLAS_PANEL_LAUNCH(VT, RPT, OT, SRP)                                                 \
-  hipLaunchKernelGGL((k_panel_spmv<VT, RPT, OT>), dim3(grid), dim3(256), 0, s, m, n,          \
-                     f.npanels, xcd_remap, f.q_lo, f.slice_base, f.slice_off, SRP,            \
-                     static_cast<const VT *>(f.pval), f.pcol, x, y, alpha, beta)
-#define SBLAS_PANEL_RPT(VT)                                                                  \
-  if (f.srp) {                                                                               \
-    if (f.rpt == 4) SBLAS_PANEL_LAUNCH(VT, 4, uint16_t, f.srp);                              \
-    else SBLAS_PANEL_LAUNCH(VT, 2, uint16_t, f.srp);                                         \
-  } else {                                                                                   \
-    if (f.rpt == 4) SBLAS_PANEL_LAUNCH(VT, 4, uint32_t, f.srp32);                            \
-    else SBLAS_PANEL_LAUNCH(VT, 2, uint32_t, f.srp32);                                       \
+  switch (val_type) {
+    case SBLAS_VAL_F64: return panel_launch_vt<double>(f, m, n, xcd_remap, x, y, alpha, beta, s);
+    case SBLAS_VAL_F32: return panel_launch_vt<float>(f, m, n, xcd_remap, x, y, alpha, beta, s);
+    default: return panel_launch_vt<uint16_t>(f, m, n, xcd_remap, x, y, alpha, beta, s);
   }
-  if (val_type == SBLAS_VAL_F64) {
-    SBLAS_PANEL_RPT(double)
-  } else if (val_type == SBLAS_VAL_F32) {
-    SBLAS_PANEL_RPT(float)
-  } else {
-    SBLAS_PANEL_RPT(uint16_t)
-  }
-#undef SBLAS_PANEL_RPT
-#undef SBLAS_PANEL_LAUNCH
-  SBLAS_HIP_TRY(hipGetLastError());
-  return SBLAS_OK;
 }
 
 }  // namespace sblas

@@ -9,6 +9,8 @@
 // the partial of a row that began in an earlier tile (head) and of a row that
 // continues into the next tile (tail).  k_tile_fixup then adds those partials
 // in tile order, so every row's sum has a fixed order: no atomics on y.
+#include <hipcub/hipcub.hpp>
+
 #include "internal.hpp"
 
 namespace sblas {
@@ -164,19 +166,13 @@ __global__ __launch_bounds__(BLOCK) void k_csr_merge_path(
 
 // ---------------------------------------------------------------------------
 // COO segmented reduction: tile b owns elements [b*T, (b+1)*T) of the
-// row-sorted arrays.  Rows with no element are written (beta*y) by the tile
-// whose first element follows them; rows after the last element by the last
-// tile.
+// row-sorted arrays.  The tile's row segments (runs of equal row ids) are
+// listed once by a workgroup prefix sum over row-start flags; sub-waves then
+// sum one segment each, so the cost follows the number of non-empty rows, not
+// the span of row ids.  Rows with no element are written (beta*y) by the
+// segment just before them, or by the tile whose first element follows them;
+// rows after the last element by the last tile.
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ int lds_lower_bound(const int *a, int n, int key) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (a[mid] < key) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
 
 template <typename VT, int ITEMS, int SUBW, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_coo_tiles(
@@ -184,8 +180,11 @@ __global__ __launch_bounds__(BLOCK) void k_coo_tiles(
     const VT *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
     double alpha, double beta, TileCarry *__restrict__ carry) {
   constexpr int T = BLOCK * ITEMS;
+  typedef hipcub::BlockScan<int, BLOCK> Scan;
   __shared__ double prod[T];
   __shared__ int rsh[T];
+  __shared__ int seg[T + 1];  // first element of each segment, then ne
+  __shared__ typename Scan::TempStorage scan_tmp;
   __shared__ double wsum[BLOCK / 64];
   const int b = blockIdx.x;
   if (b >= tiles) return;
@@ -222,22 +221,42 @@ __global__ __launch_bounds__(BLOCK) void k_coo_tiles(
     }
     return;
   }
+  // segment list: thread t flags its ITEMS consecutive elements
+  int cnt = 0;
+#pragma unroll
+  for (int i = 0; i < ITEMS; i++) {
+    const int e = t * ITEMS + i;
+    cnt += (e < ne && (e == 0 || rsh[e] != rsh[e - 1])) ? 1 : 0;
+  }
+  int pos = 0, nseg = 0;
+  Scan(scan_tmp).ExclusiveSum(cnt, pos, nseg);
+#pragma unroll
+  for (int i = 0; i < ITEMS; i++) {
+    const int e = t * ITEMS + i;
+    if (e < ne && (e == 0 || rsh[e] != rsh[e - 1])) seg[pos++] = e;
+  }
+  if (t == 0) seg[nseg] = ne;
+  __syncthreads();
   constexpr int SUBS = BLOCK / SUBW;
   const int sub = t / SUBW, lane = t % SUBW;
-  for (int r = r_first + sub; r <= r_last; r += SUBS) {
-    const int s0 = lds_lower_bound(rsh, ne, r);
-    const int e0 = lds_lower_bound(rsh, ne, r + 1);
+  for (int k = sub; k < nseg; k += SUBS) {
+    const int s0 = seg[k], e0 = seg[k + 1];
+    const int r = rsh[s0];
     double s = 0.0;
     for (int j = s0 + lane; j < e0; j += SUBW) s += prod[j];
     if constexpr (SUBW > 1) s = lanes_sum<SUBW>(s);
     if (lane == 0) {
-      if (r == r_first && head) {
+      if (k == 0 && head) {
         carry[b].head = s;
-      } else if (r == r_last && tail) {
+      } else if (k == nseg - 1 && tail) {
         carry[b].tail = s;
       } else {
         y[r] = axpby(alpha, s, beta, y[r]);
       }
+    }
+    if (k + 1 < nseg) {  // rows without elements up to the next segment's row
+      const int rn = rsh[e0];
+      for (int q = r + 1 + lane; q < rn; q += SUBW) y[q] = axpby(alpha, 0.0, beta, y[q]);
     }
   }
   if (t == 0) {

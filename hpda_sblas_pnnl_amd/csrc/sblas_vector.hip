@@ -244,6 +244,74 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
   }
 }
 
+// K right-hand sides over the row-block list (X, Y row-major, K values per
+// row).  A block of several rows gives each row SUBW lanes, K of which share
+// an element (one lane per vector); a block holding one row, or a piece of a
+// long row, spreads that row over all BLOCK/K lane groups and reduces across
+// the workgroup in a fixed order.  Rows cut by the block's element window
+// leave partials (slot 2b for the block's first row, 2b+1 for its last) for
+// the ordered merge, so a row is never the work of a single sub-wave however
+// long it is.
+template <typename VT, int K, int SUBW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_csr_blocks_mrhs(
+    const RowBlock *__restrict__ blocks, int nblocks, int xcd_remap,
+    const int *__restrict__ rp, const int *__restrict__ col, const VT *__restrict__ val,
+    const double *__restrict__ X, double *__restrict__ Y, double alpha, double beta,
+    double *__restrict__ partials) {
+  static_assert(SUBW >= K && BLOCK % 64 == 0, "lane layout");
+  __shared__ double red[BLOCK / 64][K];
+  const int b = logical_block(nblocks, xcd_remap);
+  if (b < 0) return;
+  const RowBlock rb = blocks[b];
+  const int t = threadIdx.x;
+  const int vec = t % K;
+  if (rb.row_end - rb.row_begin == 1) {
+    constexpr int G = BLOCK / K;
+    const int g = t / K;
+    double sum = 0.0;
+    for (int j = rb.lo + g; j < rb.hi; j += G)
+      sum += load_val(val, j) * X[static_cast<int64_t>(load_col(col, j)) * K + vec];
+#pragma unroll
+    for (int off = 32; off >= K; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if ((t & 63) < K) red[t >> 6][vec] = sum;
+    __syncthreads();
+    if (t < K) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < BLOCK / 64; w++) s += red[w][t];
+      const int r = rb.row_begin;
+      if (rp[r] < rb.lo || rp[r + 1] > rb.hi) {
+        partials[static_cast<int64_t>(2 * b) * K + t] = s;
+      } else {
+        double *yp = Y + static_cast<int64_t>(r) * K + t;
+        *yp = (beta == 0.0) ? alpha * s : alpha * s + beta * (*yp);
+      }
+    }
+    return;
+  }
+  constexpr int SUBS = BLOCK / SUBW;
+  constexpr int STEP = SUBW / K;
+  const int sub = t / SUBW;
+  const int grp = (t % SUBW) / K;
+  for (int r = rb.row_begin + sub; r < rb.row_end; r += SUBS) {
+    const int rs = rp[r], re = rp[r + 1];
+    const int s = max(rs, rb.lo), e = min(re, rb.hi);
+    double sum = 0.0;
+    for (int j = s + grp; j < e; j += STEP)
+      sum += load_val(val, j) * X[static_cast<int64_t>(load_col(col, j)) * K + vec];
+#pragma unroll
+    for (int off = SUBW / 2; off >= K; off >>= 1) sum += __shfl_xor(sum, off, SUBW);
+    if (grp == 0) {
+      if (rs < rb.lo || re > rb.hi) {
+        partials[static_cast<int64_t>(2 * b + (r == rb.row_begin ? 0 : 1)) * K + vec] = sum;
+      } else {
+        double *yp = Y + static_cast<int64_t>(r) * K + vec;
+        *yp = (beta == 0.0) ? alpha * sum : alpha * sum + beta * (*yp);
+      }
+    }
+  }
+}
+
 // Split rows: y[row] = alpha * (partials of the row in chunk order) + beta*y.
 __global__ void k_merge(const int *__restrict__ rows, const int *__restrict__ ptr,
                         const int *__restrict__ slots, int groups,
@@ -394,7 +462,74 @@ int launch_stream_typed(const StreamLaunch &L, const RowBlock *blocks, const int
   }
 }
 
+template <typename VT, int K>
+int launch_blocks_mrhs_k(const StreamLaunch &L, const RowBlock *blocks, const int *rp,
+                         const int *col, const VT *val, const double *X, double *Y,
+                         double alpha, double beta, double *partials, hipStream_t s) {
+  constexpr int B = kStreamBlock;
+  const dim3 grid(L.grid), block(B);
+  const int w = L.subwave < K ? K : L.subwave;
+#define SBLAS_BMRHS_CASE(W)                                                                \
+  case W:                                                                                  \
+    if constexpr (W >= K)                                                                  \
+      hipLaunchKernelGGL((k_csr_blocks_mrhs<VT, K, W, B>), grid, block, 0, s, blocks,      \
+                         L.nblocks, L.xcd_remap, rp, col, val, X, Y, alpha, beta,          \
+                         partials);                                                        \
+    break;
+  switch (w) {
+    SBLAS_BMRHS_CASE(2)
+    SBLAS_BMRHS_CASE(4)
+    SBLAS_BMRHS_CASE(8)
+    SBLAS_BMRHS_CASE(16)
+    SBLAS_BMRHS_CASE(32)
+    SBLAS_BMRHS_CASE(64)
+    default:
+      set_error(SBLAS_ERR_ARGS, "spmm kernel: bad sub-wave");
+      return SBLAS_ERR_ARGS;
+  }
+#undef SBLAS_BMRHS_CASE
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
+template <typename VT>
+int launch_blocks_mrhs_typed(int k, const StreamLaunch &L, const RowBlock *blocks,
+                             const int *rp, const int *col, const VT *val, const double *X,
+                             double *Y, double alpha, double beta, double *partials,
+                             hipStream_t s) {
+  switch (k) {
+    case 2: return launch_blocks_mrhs_k<VT, 2>(L, blocks, rp, col, val, X, Y, alpha, beta, partials, s);
+    case 4: return launch_blocks_mrhs_k<VT, 4>(L, blocks, rp, col, val, X, Y, alpha, beta, partials, s);
+    case 8: return launch_blocks_mrhs_k<VT, 8>(L, blocks, rp, col, val, X, Y, alpha, beta, partials, s);
+    case 16: return launch_blocks_mrhs_k<VT, 16>(L, blocks, rp, col, val, X, Y, alpha, beta, partials, s);
+    default:
+      set_error(SBLAS_ERR_ARGS, "spmm: k must be 1, 2, 4, 8 or 16");
+      return SBLAS_ERR_ARGS;
+  }
+}
+
 }  // namespace
+
+int launch_csr_blocks_mrhs(int val_type, int k, const StreamLaunch &L, const RowBlock *blocks,
+                           const int *rp, const int *col, const void *val, const double *X,
+                           double *Y, double alpha, double beta, double *partials,
+                           hipStream_t s) {
+  if (L.grid <= 0) return SBLAS_OK;
+  switch (val_type) {
+    case SBLAS_VAL_F64:
+      return launch_blocks_mrhs_typed(k, L, blocks, rp, col, static_cast<const double *>(val), X,
+                                      Y, alpha, beta, partials, s);
+    case SBLAS_VAL_F32:
+      return launch_blocks_mrhs_typed(k, L, blocks, rp, col, static_cast<const float *>(val), X,
+                                      Y, alpha, beta, partials, s);
+    case SBLAS_VAL_BF16:
+      return launch_blocks_mrhs_typed(k, L, blocks, rp, col, static_cast<const uint16_t *>(val),
+                                      X, Y, alpha, beta, partials, s);
+    default:
+      set_error(SBLAS_ERR_ARGS, "unknown value type");
+      return SBLAS_ERR_ARGS;
+  }
+}
 
 int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *blocks,
                       const int *rp, const int *col, const void *val, const double *x,

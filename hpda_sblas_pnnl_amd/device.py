@@ -36,7 +36,7 @@ def val_type_of(t) -> int:
 
 
 def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=0, sigma=0,
-              xcd_remap=-1, variant=0, items=0) -> Opts:
+              xcd_remap=-1, variant=0, items=0, panel_rows=0) -> Opts:
     o = Opts()
     lib().sblas_opts_default(ctypes.byref(o))
     o.kernel = _KERNELS[kernel]
@@ -49,6 +49,7 @@ def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=
     o.xcd_remap = xcd_remap
     o.variant = variant
     o.items = items
+    o.panel_rows = panel_rows
     return o
 
 

@@ -191,9 +191,10 @@ typedef struct sblas_opts {
   int sigma;       /* CSR5 nnz per lane per tile (0 = autotuned)            */
   int xcd_remap;   /* 1: contiguous row ranges per XCD; 0: off; -1: auto   */
   int variant;     /* kernel 1 form: 0 auto, 1 sub-wave per row, 2 row blocks */
-                   /* staged in LDS                                          */
+                   /* staged in LDS, 3 column panels with x staged in LDS    */
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
-  int reserved[6];
+  int panel_rows;  /* column-panel form: rows per row panel (1..2048; 0 auto) */
+  int reserved[5];
 } sblas_opts;
 
 /* Fills opts with defaults (auto kernel, C=1, v1 partition, fused launch). */
@@ -228,10 +229,13 @@ typedef struct sblas_plan_info {
   double mean_row, cv_row;
   int64_t max_row;
   int launches;      /* kernel launches per sblas_plan_spmv                 */
-  int variant;       /* kernel 1 form in use (1 sub-wave, 2 row blocks)      */
+  int variant;       /* kernel 1 form in use (1 sub-wave, 2 row blocks,     */
+                     /* 3 column panels)                                    */
   int row_blocks;    /* row blocks of the row-block form                    */
   int items;         /* elements per thread of the row-block form           */
-  int reserved[4];
+  int panel_rows;    /* column-panel form: rows per row panel               */
+  int panel_slices;  /* column-panel form: (row panel, column panel) slices */
+  int reserved[2];
 } sblas_plan_info;
 int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info);
 /* Chunk descriptors of the plan (reference rule of opts.part_mode). */

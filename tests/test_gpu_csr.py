@@ -262,6 +262,26 @@ def test_plan_spmm(oracle, k):
     assert oracle.err_normwise(dY.cpu().numpy().ravel(), ref.ravel()) <= TOL
 
 
+@pytest.mark.parametrize("k", [2, 4, 8, 16])
+@pytest.mark.parametrize("variant,chunks_", [(2, 1), (2, 3), (1, 3)])
+def test_plan_spmm_long_rows(oracle, k, variant, chunks_):
+    """Multi-RHS blocks on rows far longer than one row block (pieces merged in
+    order), empty rows and rows cut by chunk edges, beta != 0."""
+    import torch
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    rng = np.random.default_rng(60 + k)
+    m, n = 3000, 30000
+    rp, col, val = random_csr(rng, m, n, 12, empty_frac=0.3, long_rows=3, long_len=5 * 2048 + 7)
+    X = rng.uniform(-1, 1, (n, k))
+    Y0 = rng.uniform(-1, 1, (m, k))
+    drp, dcol, dval, dX, dY = _dev(rp, col, val, X, Y0)
+    plan = SpmvPlan("csr", m, n, drp, dcol, dval, kernel="vector", chunks=chunks_, variant=variant)
+    plan.spmm(dX, dY, -1.25, 0.5)
+    torch.cuda.synchronize()
+    ref = oracle.csr_spmm(m, k, val, rp, col, X, -1.25, 0.5, Y0)
+    assert oracle.err_normwise(dY.cpu().numpy().ravel(), ref.ravel()) <= TOL
+
+
 def test_plan_streams_mode_matches_fused(oracle):
     import torch
     from hpda_sblas_pnnl_amd import gen as G
