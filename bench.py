@@ -268,6 +268,40 @@ def extras_batched(log_to, count=64):
     return out
 
 
+def extras_banded_precision(A, rp, col, dx, x, y0, alpha, beta, chunks, part, plan64, y_ref,
+                            log_to):
+    """The headline matrix with FP32 and BF16 values (FP64 x, y and
+    accumulation), same chunks and partition: time per SpMV against the FP64
+    plan and normwise / max-row error against the FP64 host oracle."""
+    import torch
+    from hpda_sblas_pnnl_amd import gen as G
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    from oracle import oracle as ora
+    dev = torch.device("cuda:0")
+    dy = torch.from_numpy(y0).to(dev)
+    ms64 = _time_calls(lambda: plan64.spmv(dx, dy, alpha, beta), reps=20)
+    out = {"fp64_ms": round(ms64, 4)}
+    for name, vals, vb in (("fp32_values", A.val.astype(np.float32), 4),
+                           ("bf16_values", G.to_bf16_bits(A.val).view(np.int16), 2)):
+        dv = torch.from_numpy(vals).to(dev)
+        p = SpmvPlan("csr", A.m, A.n, rp, col, dv, chunks=chunks, part_mode=part)
+        dy = torch.from_numpy(y0).to(dev)
+        p.spmv(dx, dy, alpha, beta)
+        torch.cuda.synchronize()
+        got = dy.cpu().numpy()
+        ms = _time_calls(lambda: p.spmv(dx, dy, alpha, beta), reps=20)
+        cb = compulsory_bytes(A.m, A.n, A.nnz, vb, 1, beta != 0)
+        out[name] = {"ms": round(ms, 4), "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
+                     "GBs": round(cb / ms / 1e6, 1), "frac": round(cb / ms / 1e6 / PEAK_HBM_GBS, 4),
+                     "speedup_vs_fp64": round(ms64 / ms, 3), "variant": p.info()["variant"],
+                     "err_normwise": ora.err_normwise(got, y_ref),
+                     "err_maxrow": ora.err_maxrow(A.m, A.val, A.row_ptr, A.col, x, alpha, beta,
+                                                  y0, got, y_ref)}
+        log_to(f"[bench] banded {name}: {out[name]}")
+        del p, dv
+    return out
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
@@ -382,9 +416,17 @@ def main():
         line["cpu_baseline"] = cpu
         line["err_normwise_vs_oracle"] = ora.err_normwise(y_gpu, y_ref)
     if not args.no_extras and args.workload == "banded":
+        extras = {}
+        from oracle import oracle as ora
+        y_ref = ora.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, alpha, beta, y0,
+                             threads=len(os.sched_getaffinity(0)))
+        try:
+            extras["banded_precision"] = extras_banded_precision(
+                A, rp, col, dx, x, y0, alpha, beta, chunks, part, plan, y_ref, log)
+        except Exception as e:  # noqa: BLE001  (the main line must still print)
+            extras["banded_precision"] = {"error": f"{type(e).__name__}: {e}"}
         del plan, rp, col, val
         torch.cuda.empty_cache()
-        extras = {}
         for name, fn in (("rmat", extras_rmat), ("batched", extras_batched)):
             t0 = time.time()
             try:

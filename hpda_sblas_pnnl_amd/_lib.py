@@ -139,11 +139,11 @@ class BatchItem(ctypes.Structure):
 
 class BatchChoice(ctypes.Structure):
     _fields_ = [("kernel", _I), ("subwave", _I), ("items", _I), ("chunks", _I),
-                ("row_blocks", _I), ("pad_", _I), ("mean_row", _D), ("cv_row", _D),
+                ("row_blocks", _I), ("variant", _I), ("mean_row", _D), ("cv_row", _D),
                 ("max_row", _I64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 _lib = None
 _gen = None

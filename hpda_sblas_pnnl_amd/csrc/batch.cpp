@@ -12,12 +12,13 @@
 namespace sblas {
 namespace {
 
-// Autotuner decision for one row-length profile.  Lanes per row follow the
-// mean row length (a quarter of it, so a sub-wave strides each row about
-// four times through LDS); matrices above 2^24 elements are cut into 8 nnz
-// chunks (v1 rule) before packing row blocks, so their work items
-// interleave by chunk in the list; smaller ones keep one chunk.  Decisions
-// are cached by the rounded profile so equal-shaped matrices agree.
+// Autotuner decision for one row-length profile (the row form).  Lanes per
+// row follow the mean row length (a quarter of it, so a sub-wave strides each
+// row about four times through LDS); matrices above 2^24 elements are cut
+// into 8 nnz chunks (v1 rule) before packing row blocks; smaller ones keep
+// one chunk.  Decisions are cached by the rounded profile so equal-shaped
+// matrices agree.  Whether a matrix takes the column-panel form instead is
+// decided from its column structure when the batch is created.
 struct Choice {
   int subwave = 16, items = 8, chunks = 1;
 };
@@ -51,9 +52,11 @@ struct sblas_batch_s {
   std::vector<sblas::BatchMatrix> mats;
   std::vector<sblas::BatchItem> items;
   std::vector<sblas_batch_choice> choices;
+  std::vector<sblas::PanelFormat> panels;  // column-panel forms, one per matrix (may be empty)
   sblas::BatchMatrix *d_mats = nullptr;
   sblas::BatchItem *d_items = nullptr;
   double *d_partials = nullptr;
+  int *d_ctr = nullptr;
   int *d_gmat = nullptr, *d_grow = nullptr, *d_gptr = nullptr, *d_gslot = nullptr;
   int groups = 0;
   int grid = 0;
@@ -67,6 +70,8 @@ struct sblas_batch_s {
     if (graph) (void)hipGraphDestroy(graph);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (void *p : bufs) (void)hipFree(p);
+    for (sblas::PanelFormat &f : panels)
+      for (void *p : f.buffers) (void)hipFree(p);
   }
   template <typename T>
   int upload(T **dst, const std::vector<T> &src) {
@@ -86,8 +91,8 @@ struct sblas_batch_s {
     return SBLAS_OK;
   }
   int enqueue(hipStream_t s) {
-    int st = sblas::launch_batch_rowblocks(items_per_thread, grid, static_cast<int>(items.size()),
-                                           d_items, d_mats, d_partials, s);
+    int st = sblas::launch_batch(items_per_thread, grid, static_cast<int>(items.size()), d_items,
+                                 d_mats, d_partials, d_ctr, s);
     if (st != SBLAS_OK) return st;
     return sblas::launch_batch_merge(groups, d_gmat, d_grow, d_gptr, d_gslot, d_mats, d_partials,
                                      s);
@@ -115,6 +120,15 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     return code;
   };
   std::vector<int> gmat, grow, gptr(1, 0), gslot;
+  // work items of each matrix in its own order, before the list is sorted
+  struct Pending {
+    BatchItem item;
+    int64_t work;        // elements
+    int lo, hi;          // element window
+    int row_begin, row_end;
+  };
+  std::vector<std::vector<Pending>> pending(count);
+  std::vector<std::vector<int>> h_rps(count);
   for (int i = 0; i < count; i++) {
     const sblas_batch_item &it = items[i];
     if (it.m < 0 || it.n < 0 || it.nnz < 0 || it.nnz > INT32_MAX - 4096 || !it.d_row_ptr ||
@@ -163,24 +177,93 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     M.val_type = it.val_type;
     M.subwave = c.subwave;
     M.m = it.m;
-    b->mats.push_back(M);
+    M.n = it.n;
 
+    // column-panel form when the matrix's column structure pays for it (one
+    // chunk: rows are never cut; 16-bit slice offsets only)
+    const int want = opts ? opts->variant : 0;
+    b->panels.emplace_back();
+    PanelFormat &f = b->panels.back();
+    bool panel = false;
+    if (it.m > 0 && it.nnz > 0 && want != 1 && want != 2 && (want == 3 || rs.max_row <= 8192)) {
+      const std::vector<ExecChunk> one{ExecChunk{0, it.nnz, 0, it.m}};
+      const int rows = (opts && opts->panel_rows > 0) ? opts->panel_rows : 1024;
+      int st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col,
+                           it.d_val, one, h_rp.data(), want == 3, 1, f, &panel, nullptr);
+      if (st == SBLAS_OK && panel && !f.srp) panel = false;  // slices beyond 16-bit offsets
+      if (st == SBLAS_ERR_UNSUPPORTED && want != 3) {
+        clear_error();
+        st = SBLAS_OK;
+        panel = false;
+      }
+      if (st != SBLAS_OK) return fail(st);
+      if (!panel) {
+        for (void *q : f.buffers) (void)hipFree(q);
+        f = PanelFormat{};
+      }
+    }
+    if (panel) {
+      M.pinfo = f.info;
+      M.slice_base = f.slice_base;
+      M.slice_off = f.slice_off;
+      M.srp = f.srp;
+      M.pval = f.pval;
+      M.pcol = f.pcol;
+      M.prows = f.rows;
+      c.chunks = 1;
+    }
+    b->mats.push_back(M);
     if (it.m == 0) {
       b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks, 0,
-                                              0, rs.mean, rs.cv, rs.max_row});
+                                              2, rs.mean, rs.cv, rs.max_row});
       continue;
     }
-    const std::vector<int64_t> bounds =
-        element_bounds(SBLAS_PART_V1, it.m, h_rp.data(), it.nnz, c.chunks, 1);
-    const std::vector<ExecChunk> ex = exec_chunks_from_bounds(it.m, h_rp.data(), bounds);
-    const std::vector<RowBlock> rb = build_row_blocks(ex, h_rp.data(), kStreamBlock * c.items);
-    const int first_item = static_cast<int>(b->items.size());
-    for (const RowBlock &r : rb)
-      b->items.push_back(BatchItem{i, r.row_begin, r.row_end, r.lo, r.hi, {0, 0, 0}});
-    // split rows of this matrix: partial slots 2*item (+1 for a block's last row)
-    for (size_t q = 0; q < rb.size(); q++) {
-      const RowBlock &r = rb[q];
-      const int item = first_item + static_cast<int>(q);
+    std::vector<Pending> &mine = pending[i];
+    if (panel) {
+      for (size_t q = 0; q < f.hpanels.size(); q++) {
+        const ExecChunk &e = f.hpanels[q];
+        mine.push_back(Pending{BatchItem{i, kBatchPanel, static_cast<int>(q), 0, 0, 0, {0, 0}},
+                               e.hi - e.lo, static_cast<int>(e.lo), static_cast<int>(e.hi),
+                               e.row_begin, e.row_end});
+      }
+    } else {
+      const std::vector<int64_t> bounds =
+          element_bounds(SBLAS_PART_V1, it.m, h_rp.data(), it.nnz, c.chunks, 1);
+      const std::vector<ExecChunk> ex = exec_chunks_from_bounds(it.m, h_rp.data(), bounds);
+      for (const RowBlock &r : build_row_blocks(ex, h_rp.data(), kStreamBlock * c.items))
+        mine.push_back(Pending{BatchItem{i, kBatchRowBlock, r.row_begin, r.row_end, r.lo, r.hi,
+                                         {0, 0}},
+                               r.hi - r.lo, r.lo, r.hi, r.row_begin, r.row_end});
+    }
+    b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks,
+                                            static_cast<int>(mine.size()), panel ? 3 : 2, rs.mean,
+                                            rs.cv, rs.max_row});
+    h_rps[i] = std::move(h_rp);
+  }
+  // list order: row panels heaviest first, then row blocks in matrix order
+  std::vector<std::pair<int, int>> order;  // (matrix, index in its list)
+  for (int i = 0; i < count; i++)
+    for (int q = 0; q < static_cast<int>(pending[i].size()); q++)
+      if (pending[i][q].item.kind == kBatchPanel) order.emplace_back(i, q);
+  std::stable_sort(order.begin(), order.end(), [&](const auto &a, const auto &c2) {
+    return pending[a.first][a.second].work > pending[c2.first][c2.second].work;
+  });
+  for (int i = 0; i < count; i++)
+    for (int q = 0; q < static_cast<int>(pending[i].size()); q++)
+      if (pending[i][q].item.kind == kBatchRowBlock) order.emplace_back(i, q);
+  std::vector<std::vector<int>> where(count);
+  for (int i = 0; i < count; i++) where[i].assign(pending[i].size(), -1);
+  for (const auto &o : order) {
+    where[o.first][o.second] = static_cast<int>(b->items.size());
+    b->items.push_back(pending[o.first][o.second].item);
+  }
+  // rows cut by an item's element window: partial slots 2w (first row) and
+  // 2w+1 (last row), listed per row in the matrix's own item order
+  for (int i = 0; i < count; i++) {
+    const std::vector<int> &h_rp = h_rps[i];
+    for (size_t q = 0; q < pending[i].size(); q++) {
+      const Pending &r = pending[i][q];
+      const int w = where[i][q];
       auto split = [&](int row) { return h_rp[row] < r.lo || h_rp[row + 1] > r.hi; };
       auto add = [&](int row, int slot) {
         if (!grow.empty() && gmat.back() == i && grow.back() == row) {
@@ -194,12 +277,10 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
         }
       };
       const int first = r.row_begin, last = r.row_end - 1;
-      if (split(first)) add(first, 2 * item);
-      if (last != first && split(last)) add(last, 2 * item + 1);
+      if (first > last) continue;
+      if (split(first)) add(first, 2 * w);
+      if (last != first && split(last)) add(last, 2 * w + 1);
     }
-    b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks,
-                                            static_cast<int>(rb.size()), 0, rs.mean, rs.cv,
-                                            rs.max_row});
   }
   b->groups = static_cast<int>(grow.size());
   int st;
@@ -212,11 +293,13 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     return fail(st);
   std::vector<double> zeros(2 * std::max<size_t>(b->items.size(), 1), 0.0);
   if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);
+  if ((st = b->upload(&b->d_ctr, std::vector<int>{0, 0})) != SBLAS_OK) return fail(st);
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // a few resident workgroups per CU, never more than there are items
-  b->grid = std::min<int>(static_cast<int>(b->items.size()), std::max(1, cus) * 4);
+  // the resident workgroups, never more than there are items
+  const int per_cu = batch_occupancy(b->items_per_thread);
+  b->grid = std::min<int>(static_cast<int>(b->items.size()), std::max(1, cus) * per_cu);
   *out = b;
   return SBLAS_OK;
 }

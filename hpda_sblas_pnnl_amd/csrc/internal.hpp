@@ -125,12 +125,20 @@ int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha
 // column-panel form of kernel 1 (sblas_panel.hip)
 constexpr int kPanelCols = 4096;  // x panel staged in LDS (32 KB of FP64)
 constexpr int kPanelRowsMax = 2048;  // rows per row panel (8 per thread)
+// One row panel: rows [row0, row0 + nrow) of one execution chunk, its first
+// column panel, and flags: bit 0 = the first row is cut by the chunk's
+// element window (partial slot 2p), bit 1 = the last row is (slot 2p+1).
+struct PanelInfo {
+  int row0, nrow;
+  int q_lo;
+  int flags;
+};
 struct PanelFormat {
-  int rows = 0;       // rows per row panel
+  int rows = 0;       // rows per row panel (at most)
   int rpt = 1;        // rows per thread: ceil(rows/256) rounded to 1, 2, 4, 6 or 8
   int npanels = 0, nslices = 0;
   double avg_panels_per_row_panel = 0, avg_slice = 0;
-  int *q_lo = nullptr;          // first column panel of each row panel
+  PanelInfo *info = nullptr;    // per row panel
   int *slice_base = nullptr;    // first slice of each row panel (npanels+1)
   int64_t *slice_off = nullptr; // element offset of each slice (nslices+1)
   // per slice: R+1 row offsets inside the slice, 16-bit when every slice
@@ -139,18 +147,21 @@ struct PanelFormat {
   uint32_t *srp32 = nullptr;
   void *pval = nullptr;         // values, slice-major
   uint16_t *pcol = nullptr;     // column offsets inside the column panel
+  double *partials = nullptr;   // 2 slots per row panel
+  std::vector<ExecChunk> hpanels;  // host copy: element window and rows of each panel
   std::vector<void *> buffers;
 };
 int panel_rpt(int rows);
 // Rows per row panel so the panels fill whole rounds of resident workgroups.
 int panel_rows_for(int val_type, int m, int device);
-// Builds the format on the device with row panels of `rows` rows.  With
+// Builds the format on the device: each execution chunk of `ex` cut into row
+// panels of at most `rows` rows (h_rp: host copy of the row pointers).  With
 // force == false it stops after the per-panel column ranges and sets
 // *chosen = false when the form would not help (few row panels, sparse
 // slices, or rows whose x window is narrow).
 int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp, const int *col,
-                const void *val, bool force, int min_panels, PanelFormat &f, bool *chosen,
-                hipStream_t s);
+                const void *val, const std::vector<ExecChunk> &ex, const int *h_rp,
+                bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s);
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
                       const double *x, double *y, double alpha, double beta, hipStream_t s);
 
@@ -164,17 +175,34 @@ struct BatchMatrix {
   double alpha, beta;
   int val_type;
   int subwave;  // lanes per row in the LDS reduction (1..64)
-  int m;
+  int m, n;
+  // column-panel form (pinfo == nullptr: the matrix runs as row blocks)
+  const PanelInfo *pinfo;
+  const int *slice_base;
+  const int64_t *slice_off;
+  const uint16_t *srp;
+  const void *pval;
+  const uint16_t *pcol;
+  int prows;    // rows per row panel
   int pad_;
 };
-struct BatchItem {  // one row block of one matrix
+constexpr int kBatchRowBlock = 0;
+constexpr int kBatchPanel = 1;
+struct BatchItem {  // one row block, or one row panel, of one matrix
   int mat;
-  int row_begin, row_end;
-  int lo, hi;
-  int pad_[3];
+  int kind;     // kBatchRowBlock or kBatchPanel
+  int row_begin, row_end;  // row block: rows; row panel: row_begin = panel index
+  int lo, hi;              // row block: element window
+  int pad_[2];
 };
-int launch_batch_rowblocks(int items_per_thread, int grid, int nitems, const BatchItem *items,
-                           const BatchMatrix *mats, double *partials, hipStream_t s);
+// Persistent launch over all items: workgroups take items from a counter in
+// ctr[0] (ctr[1] counts finished workgroups; the last one resets both), so
+// heavy items listed first spread over the GPU.  Each item's arithmetic does
+// not depend on which workgroup runs it.
+int launch_batch(int items_per_thread, int grid, int nitems, const BatchItem *items,
+                 const BatchMatrix *mats, double *partials, int *ctr, hipStream_t s);
+// Resident workgroups per CU of the batched kernel.
+int batch_occupancy(int items_per_thread);
 int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,
                        const int *gslot, const BatchMatrix *mats, const double *partials,
                        hipStream_t s);

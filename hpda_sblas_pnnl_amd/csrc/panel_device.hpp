@@ -1,0 +1,168 @@
+// panel_device.hpp -- device body of the column-panel form of kernel 1, shared
+// by the single-matrix launch (sblas_panel.hip) and the batched persistent
+// launch (sblas_batch.hip).  Layout and order of operations are described in
+// sblas_panel.hip.
+#pragma once
+
+#include "internal.hpp"
+
+namespace sblas {
+namespace panel {
+
+__device__ __forceinline__ double pv(const double *p, int64_t j) {
+  return __builtin_nontemporal_load(p + j);
+}
+__device__ __forceinline__ double pv(const float *p, int64_t j) {
+  return static_cast<double>(__builtin_nontemporal_load(p + j));
+}
+__device__ __forceinline__ double pv(const uint16_t *p, int64_t j) {
+  const uint32_t h = __builtin_nontemporal_load(p + j);
+  return static_cast<double>(__uint_as_float(h << 16));
+}
+
+// Separately rounded multiply and add.  Contraction is switched off inside
+// these bodies, so a product never fuses with the add that consumes it (the
+// header intrinsics are plain * and + and would fuse under -ffp-contract=fast).
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
+constexpr int kItems = 8;               // elements per thread per chunk
+constexpr int kChunk = 256 * kItems;    // elements per chunk of a slice
+constexpr int kLdsDoubles = kPanelCols + kChunk;  // x panel + products
+
+// Current slice of a row panel and the first element of the chunk worked on.
+struct Cursor {
+  int s, k0, len;
+  int64_t base;
+};
+
+// Moves c to the first chunk of the first non-empty slice at or after s.
+__device__ __forceinline__ void seek(Cursor &c, int s, int s1,
+                                     const int64_t *__restrict__ slice_off) {
+  c.k0 = 0;
+  c.s = s;
+  c.len = 0;
+  c.base = 0;
+  for (; c.s < s1; c.s++) {
+    c.base = slice_off[c.s];
+    c.len = static_cast<int>(slice_off[c.s + 1] - c.base);
+    if (c.len > 0) return;
+  }
+}
+
+// Row panel p of one matrix, by one workgroup of 256 threads.  lds holds
+// kLdsDoubles doubles; slots[0] / slots[1] receive the partial sums of a
+// first / last row that the panel's element window cuts (PanelInfo flags).
+// Software-pipelined: the next chunk's values and column offsets are loaded
+// while the current chunk's rows are summed and the next slice's x is staged.
+// The workgroup leaves through a barrier, so LDS may be reused right after.
+template <typename VT, int RPT, typename OT>
+__device__ __forceinline__ void row_panel(
+    int p, int n, int R, const PanelInfo &pi, const int *__restrict__ slice_base,
+    const int64_t *__restrict__ slice_off, const OT *__restrict__ srp,
+    const VT *__restrict__ pval, const uint16_t *__restrict__ pcol,
+    const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
+    double *__restrict__ slots, double *lds) {
+  constexpr int W = kPanelCols;
+  double *xs = lds;
+  double *prod = lds + W;
+  const int t = threadIdx.x;
+  const int s0 = slice_base[p], s1 = slice_base[p + 1];
+  const int qb = pi.q_lo;
+  double acc[RPT];
+  int b[RPT], e[RPT];
+  double v[kItems];
+  int l[kItems];
+#pragma unroll
+  for (int j = 0; j < RPT; j++) acc[j] = 0.0;
+
+  // x[q*W, (q+1)*W) of slice s into LDS
+  auto stage_x = [&](int s) {
+    const int c0 = (qb + (s - s0)) * W;
+#pragma unroll
+    for (int i = 0; i < W / 256; i++) {
+      const int c = c0 + t + i * 256;
+      const double xv = x[min(c, n - 1)];  // no load under a branch
+      xs[t + i * 256] = (c < n) ? xv : 0.0;
+    }
+  };
+  // element ranges [b, e) of this thread's rows inside slice s
+  auto row_ranges = [&](int s) {
+    const OT *rps = srp + static_cast<int64_t>(s) * (R + 1);
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      const int rl = t + 256 * j;
+      const bool in = rl < R;
+      const int lo = rps[in ? rl : 0], hi = rps[in ? rl + 1 : 0];  // unconditional loads
+      b[j] = in ? lo : 0;
+      e[j] = in ? hi : 0;
+    }
+  };
+  // values and column offsets of a chunk: consecutive threads on consecutive
+  // elements (whole lines per load instruction), all issued back to back
+  auto load_chunk = [&](const Cursor &c) {
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+      const int64_t k = c.base + min(c.k0 + t + i * 256, c.len - 1);
+      v[i] = pv(pval, k);
+      l[i] = __builtin_nontemporal_load(pcol + k);
+    }
+  };
+
+  Cursor cur;
+  seek(cur, s0, s1, slice_off);
+  if (cur.s < s1) {
+    stage_x(cur.s);
+    row_ranges(cur.s);
+    load_chunk(cur);
+  }
+  __syncthreads();
+  while (cur.s < s1) {
+    // products of the current chunk (x of its slice is in LDS)
+#pragma unroll
+    for (int i = 0; i < kItems; i++) prod[t + i * 256] = mul_rn(v[i], xs[l[i]]);
+    Cursor nxt = cur;
+    nxt.k0 += kChunk;
+    if (nxt.k0 >= cur.len) seek(nxt, cur.s + 1, s1, slice_off);
+    // the next chunk's loads are in flight during the row sums below
+    if (nxt.s < s1) load_chunk(nxt);
+    __syncthreads();  // products visible; x of the current slice no longer read
+    const bool new_slice = nxt.s != cur.s && nxt.s < s1;
+    if (new_slice) stage_x(nxt.s);
+    // each row adds its products of this chunk in element order
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      const int lo = max(b[j], cur.k0), hi = min(e[j], cur.k0 + kChunk);
+      double a = acc[j];
+      for (int k = lo; k < hi; k++) a = add_rn(a, prod[k - cur.k0]);
+      acc[j] = a;
+    }
+    if (new_slice) row_ranges(nxt.s);
+    __syncthreads();  // products consumed; x of the next slice visible
+    cur = nxt;
+  }
+#pragma unroll
+  for (int j = 0; j < RPT; j++) {
+    const int rl = t + 256 * j;
+    if (rl < pi.nrow) {
+      const int r = pi.row0 + rl;
+      if (rl == 0 && (pi.flags & 1)) {
+        slots[0] = acc[j];
+      } else if (rl == pi.nrow - 1 && (pi.flags & 2)) {
+        slots[1] = acc[j];
+      } else {
+        y[r] = (beta == 0.0) ? mul_rn(alpha, acc[j])
+                             : add_rn(mul_rn(alpha, acc[j]), mul_rn(beta, y[r]));
+      }
+    }
+  }
+}
+
+}  // namespace panel
+}  // namespace sblas

@@ -420,6 +420,12 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     if (st != SBLAS_OK) return st;
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
     launches++;
+    if (pgroups > 0 && merge_enabled) {
+      st = launch_merge(d_pmrows, d_pmptr, d_pmslots, pgroups, panel.partials, 1, Y, alpha, beta,
+                        s);
+      if (st != SBLAS_OK) return st;
+      launches++;
+    }
     return SBLAS_OK;
   }
   if (variant == 2 && k == 1 && streams.empty()) {
@@ -579,15 +585,18 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
   if ((st = setup_vector(h_rp.data())) != SBLAS_OK) return st;
   if (kernel == SBLAS_KERNEL_MERGE) return setup_tiles();
   if (kernel == SBLAS_KERNEL_CSR5) return setup_csr5(h_rp);
-  if (variant == 2 && (opts.variant == 0 || opts.variant == 3) && opts.streams <= 1 && nnz > 0) {
+  // the panel form sums each row in one thread: not for matrices with very long rows
+  const bool panel_ok = opts.variant == 3 || stats.max_row <= 8192;
+  if (variant == 2 && (opts.variant == 0 || opts.variant == 3) && opts.streams <= 1 && nnz > 0 &&
+      panel_ok) {
     // column-panel form: forced by variant 3, else taken when it pays
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     const int min_panels = 2 * std::max(cus, 1);
     const int rows = opts.panel_rows > 0 ? opts.panel_rows : panel_rows_for(val_type, m, device);
     bool chosen = false;
-    st = panel_build(val_type, rows, m, n, nnz, a0, a1, val, opts.variant == 3, min_panels, panel,
-                     &chosen, nullptr);
+    st = panel_build(val_type, rows, m, n, nnz, a0, a1, val, exec, h_rp.data(),
+                     opts.variant == 3, min_panels, panel, &chosen, nullptr);
     if (st == SBLAS_ERR_UNSUPPORTED && opts.variant == 0) {
       // a slice too large for 16-bit offsets: keep the row-block form
       for (void *p : panel.buffers) (void)hipFree(p);
@@ -603,6 +612,13 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
       int remap = opts.xcd_remap;
       if (remap < 0) remap = env_int("SBLAS_XCD_REMAP", 1);
       vl.xcd_remap = remap ? 1 : 0;
+      // rows cut by chunk edges: partials of their panels, merged in panel order
+      std::vector<int> mrows, mptr, mslots;
+      merge_lists(panel.hpanels, h_rp.data(), mrows, mptr, mslots);
+      pgroups = static_cast<int>(mrows.size());
+      if ((st = upload(*this, &d_pmrows, mrows)) != SBLAS_OK) return st;
+      if ((st = upload(*this, &d_pmptr, mptr)) != SBLAS_OK) return st;
+      if ((st = upload(*this, &d_pmslots, mslots)) != SBLAS_OK) return st;
       return setup_stream(h_rp.data());  // row blocks serve multi-RHS products
     }
   }
@@ -617,14 +633,17 @@ bool Plan::uses_tiles(int k) const {
 
 bool Plan::has_merge(int k) const {
   if (uses_tiles(k)) return tiles > 0;
-  if (variant == 3 && k == 1 && streams.empty()) return false;  // rows never cut
+  if (variant == 3 && k == 1 && streams.empty()) return pgroups > 0;
   if (variant >= 2 && streams.empty()) return bgroups > 0;  // row blocks (any k)
   return ngroups > 0;
 }
 
 int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) {
   if (uses_tiles(k)) return tiles > 0 ? launch_tile_fixup(tiles, d_carry, Y, alpha, beta, s) : SBLAS_OK;
-  if (variant == 3 && k == 1 && streams.empty()) return SBLAS_OK;
+  if (variant == 3 && k == 1 && streams.empty())
+    return pgroups > 0 ? launch_merge(d_pmrows, d_pmptr, d_pmslots, pgroups, panel.partials, 1, Y,
+                                      alpha, beta, s)
+                       : SBLAS_OK;
   if (variant >= 2 && streams.empty())
     return bgroups > 0 ? launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, k, Y,
                                       alpha, beta, s)

@@ -69,8 +69,10 @@ struct Plan {
   int *d_tile_row = nullptr;
   uint32_t *d_tile_flags = nullptr;
 
-  // column-panel form of kernel 1 (variant 3)
+  // column-panel form of kernel 1 (variant 3); rows cut by chunk edges
   PanelFormat panel;
+  int *d_pmrows = nullptr, *d_pmptr = nullptr, *d_pmslots = nullptr;
+  int pgroups = 0;
 
   // CSR5-style format (kernel 3)
   uint32_t *d_c5_tile_nz = nullptr, *d_c5_flags = nullptr;

@@ -1,15 +1,17 @@
 // sblas_batch.hip -- many matrices of uneven sizes in one persistent launch.
 //
-// The work of every matrix is cut into row blocks (the LDS-staged form of
-// kernel 1) and all blocks of all matrices form one list.  A persistent grid
-// of a few workgroups per CU walks the list with a fixed stride (workgroup g
-// takes items g, g + grid, g + 2*grid, ...), so the assignment -- and every
-// sum -- is the same on every run; no counters or atomics are involved.
-// Each item carries its matrix's parameters (arrays, value type, lanes per
-// row chosen by the autotuner), so matrices with different row-length
-// profiles share the launch.  Pieces of rows longer than one block leave
-// partials that one merge launch combines in block order.
+// Every matrix is cut into work items of the form its autotuner choice
+// names: row panels of the column-panel form (matrices whose rows draw x from
+// a wide window) or row blocks of the LDS-staged row form, and all items of
+// all matrices form one list, heaviest first.  A persistent grid sized to
+// the resident workgroups takes items from a device counter, so the launch
+// balances itself across the GPU; an item's arithmetic (and so every sum) is
+// the same whichever workgroup runs it.  Each item carries its matrix's
+// parameters (arrays, value type, lanes per row), so matrices with different
+// profiles share the launch.  Rows cut by an item's element window leave
+// partials that one merge launch combines in item order.
 #include "internal.hpp"
+#include "panel_device.hpp"
 
 namespace sblas {
 namespace {
@@ -26,89 +28,150 @@ __device__ __forceinline__ double b_axpby(double alpha, double s, double beta, d
   return (beta == 0.0) ? alpha * s : alpha * s + beta * yv;
 }
 
+// One row block (the LDS-staged form of kernel 1): products of the block's
+// element window into LDS, sub-wave row sums, y written (or the partials of
+// rows the window cuts).  lds holds BLOCK*ITEMS + kStreamRowsMax doubles.
 template <int ITEMS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batch_rowblocks(
-    int nitems, const BatchItem *__restrict__ items, const BatchMatrix *__restrict__ mats,
-    double *__restrict__ partials) {
+__device__ __forceinline__ void row_block(int w, const BatchItem &it, const BatchMatrix &M,
+                                          double *__restrict__ partials, double *lds,
+                                          double *wsum) {
   constexpr int CAP = BLOCK * ITEMS;
-  __shared__ double prod[CAP];
-  __shared__ double ysh[kStreamRowsMax];
-  __shared__ double wsum[BLOCK / 64];
+  double *prod = lds;
+  double *ysh = lds + CAP;
   const int t = threadIdx.x;
-  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-    const BatchItem it = items[w];
-    const BatchMatrix &M = mats[it.mat];
-    const int *__restrict__ rp = M.rp;
-    const int *__restrict__ col = M.col;
-    const double *__restrict__ x = M.x;
-    double *__restrict__ y = M.y;
-    const int lo = it.lo, hi = it.hi;
-    double p[ITEMS];
-    if (hi > lo) {
-      double v[ITEMS];
-      int c[ITEMS];
-#pragma unroll
-      for (int k = 0; k < ITEMS; k++) {
-        const int e = min(lo + t + k * BLOCK, hi - 1);
-        v[k] = b_val(M.val, M.val_type, e);
-        c[k] = __builtin_nontemporal_load(col + e);
-      }
-      double xv[ITEMS];
-#pragma unroll
-      for (int k = 0; k < ITEMS; k++) xv[k] = x[c[k]];
-#pragma unroll
-      for (int k = 0; k < ITEMS; k++) p[k] = (lo + t + k * BLOCK < hi) ? v[k] * xv[k] : 0.0;
-    } else {
-#pragma unroll
-      for (int k = 0; k < ITEMS; k++) p[k] = 0.0;
-    }
-    const int nrows = it.row_end - it.row_begin;
-    if (nrows == 1 && (rp[it.row_begin] < lo || rp[it.row_begin + 1] > hi)) {
-      double s = 0.0;
-#pragma unroll
-      for (int k = 0; k < ITEMS; k++) s += p[k];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if ((t & 63) == 0) wsum[t >> 6] = s;
-      __syncthreads();
-      if (t == 0) {
-        double tot = 0.0;
-#pragma unroll
-        for (int q = 0; q < BLOCK / 64; q++) tot += wsum[q];
-        partials[2 * w] = tot;
-      }
-      __syncthreads();
-      continue;
-    }
+  const int *__restrict__ rp = M.rp;
+  const int *__restrict__ col = M.col;
+  const double *__restrict__ x = M.x;
+  double *__restrict__ y = M.y;
+  const int lo = it.lo, hi = it.hi;
+  double p[ITEMS];
+  if (hi > lo) {
+    double v[ITEMS];
+    int c[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-      const int e = t + k * BLOCK;
-      if (e < hi - lo) prod[e] = p[k];
+      const int e = min(lo + t + k * BLOCK, hi - 1);
+      v[k] = b_val(M.val, M.val_type, e);
+      c[k] = __builtin_nontemporal_load(col + e);
+    }
+    double xv[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) xv[k] = x[c[k]];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) p[k] = (lo + t + k * BLOCK < hi) ? v[k] * xv[k] : 0.0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) p[k] = 0.0;
+  }
+  const int nrows = it.row_end - it.row_begin;
+  if (nrows == 1 && (rp[it.row_begin] < lo || rp[it.row_begin + 1] > hi)) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) s += p[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((t & 63) == 0) wsum[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+      double tot = 0.0;
+#pragma unroll
+      for (int q = 0; q < BLOCK / 64; q++) tot += wsum[q];
+      partials[2 * w] = tot;
     }
     __syncthreads();
-    const int sub = M.subwave;  // 1..64, power of two
-    const int subs = BLOCK / sub;
-    const int g = t / sub, lane = t % sub;
-    for (int i = g; i < nrows; i += subs) {
-      const int r = it.row_begin + i;
-      const int s0 = max(rp[r], lo) - lo, e0 = min(rp[r + 1], hi) - lo;
-      double s = 0.0;
-      for (int j = s0 + lane; j < e0; j += sub) s += prod[j];
-      for (int off = sub >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if (lane == 0) ysh[i] = s;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) {
+    const int e = t + k * BLOCK;
+    if (e < hi - lo) prod[e] = p[k];
+  }
+  __syncthreads();
+  const int sub = M.subwave;  // 1..64, power of two
+  const int subs = BLOCK / sub;
+  const int g = t / sub, lane = t % sub;
+  for (int i = g; i < nrows; i += subs) {
+    const int r = it.row_begin + i;
+    const int s0 = max(rp[r], lo) - lo, e0 = min(rp[r + 1], hi) - lo;
+    double s = 0.0;
+    for (int j = s0 + lane; j < e0; j += sub) s += prod[j];
+    for (int off = sub >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) ysh[i] = s;
+  }
+  __syncthreads();
+  for (int i = t; i < nrows; i += BLOCK) {
+    const int r = it.row_begin + i;
+    const double s = ysh[i];
+    const bool edge = (i == 0 || i == nrows - 1);
+    if (edge && (rp[r] < lo || rp[r + 1] > hi)) {
+      partials[2 * w + (i == 0 ? 0 : 1)] = s;
+    } else {
+      y[r] = b_axpby(M.alpha, s, M.beta, y[r]);
     }
+  }
+  __syncthreads();
+}
+
+// One row panel of a matrix in column-panel form (16-bit slice offsets).
+__device__ __forceinline__ void row_panel_any(int w, const BatchItem &it, const BatchMatrix &M,
+                                              double *__restrict__ partials, double *lds) {
+  constexpr int RPT = kPanelRowsMax / 256;
+  const int p = it.row_begin;
+  const PanelInfo pi = M.pinfo[p];
+  switch (M.val_type) {
+    case SBLAS_VAL_F64:
+      panel::row_panel<double, RPT, uint16_t>(
+          p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
+          static_cast<const double *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
+          partials + 2 * w, lds);
+      break;
+    case SBLAS_VAL_F32:
+      panel::row_panel<float, RPT, uint16_t>(
+          p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
+          static_cast<const float *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
+          partials + 2 * w, lds);
+      break;
+    default:
+      panel::row_panel<uint16_t, RPT, uint16_t>(
+          p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
+          static_cast<const uint16_t *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
+          partials + 2 * w, lds);
+      break;
+  }
+}
+
+template <int ITEMS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__restrict__ items,
+                                                 const BatchMatrix *__restrict__ mats,
+                                                 double *__restrict__ partials,
+                                                 int *__restrict__ ctr) {
+  static_assert(BLOCK == 256, "the panel body assumes 256 threads");
+  constexpr int LDS_RB = BLOCK * ITEMS + kStreamRowsMax;
+  constexpr int LDS = LDS_RB > panel::kLdsDoubles ? LDS_RB : panel::kLdsDoubles;
+  __shared__ double lds[LDS];
+  __shared__ double wsum[BLOCK / 64];
+  __shared__ int next;
+  const int t = threadIdx.x;
+  for (;;) {
+    if (t == 0) next = atomicAdd(&ctr[0], 1);
     __syncthreads();
-    for (int i = t; i < nrows; i += BLOCK) {
-      const int r = it.row_begin + i;
-      const double s = ysh[i];
-      const bool edge = (i == 0 || i == nrows - 1);
-      if (edge && (rp[r] < lo || rp[r + 1] > hi)) {
-        partials[2 * w + (i == 0 ? 0 : 1)] = s;
-      } else {
-        y[r] = b_axpby(M.alpha, s, M.beta, y[r]);
-      }
+    const int w = next;
+    __syncthreads();  // every thread has read `next` before it is taken again
+    if (w >= nitems) break;  // the counter only grows: every workgroup gets here
+    const BatchItem it = items[w];
+    const BatchMatrix &M = mats[it.mat];
+    if (it.kind == kBatchPanel)
+      row_panel_any(w, it, M, partials, lds);
+    else
+      row_block<ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+  }
+  if (t == 0) {
+    // the last workgroup to finish rewinds the counters for the next launch
+    const int done = atomicAdd(&ctr[1], 1);
+    if (done == static_cast<int>(gridDim.x) - 1) {
+      atomicExch(&ctr[0], 0);
+      atomicExch(&ctr[1], 0);
     }
-    __syncthreads();
   }
 }
 
@@ -128,18 +191,28 @@ __global__ void k_batch_merge(int groups, const int *__restrict__ gmat,
 
 }  // namespace
 
-int launch_batch_rowblocks(int items_per_thread, int grid, int nitems, const BatchItem *items,
-                           const BatchMatrix *mats, double *partials, hipStream_t s) {
+int launch_batch(int items_per_thread, int grid, int nitems, const BatchItem *items,
+                 const BatchMatrix *mats, double *partials, int *ctr, hipStream_t s) {
   if (nitems <= 0 || grid <= 0) return SBLAS_OK;
   constexpr int B = kStreamBlock;
   if (items_per_thread == 16)
-    hipLaunchKernelGGL((k_batch_rowblocks<16, B>), dim3(grid), dim3(B), 0, s, nitems, items,
-                       mats, partials);
+    hipLaunchKernelGGL((k_batch<16, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats, partials,
+                       ctr);
   else
-    hipLaunchKernelGGL((k_batch_rowblocks<8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats,
-                       partials);
+    hipLaunchKernelGGL((k_batch<8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats, partials,
+                       ctr);
   SBLAS_HIP_TRY(hipGetLastError());
   return SBLAS_OK;
+}
+
+int batch_occupancy(int items_per_thread) {
+  int per_cu = 0;
+  const void *fn = items_per_thread == 16 ? reinterpret_cast<const void *>(&k_batch<16, kStreamBlock>)
+                                          : reinterpret_cast<const void *>(&k_batch<8, kStreamBlock>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kStreamBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return per_cu;
 }
 
 int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,

@@ -62,9 +62,11 @@ def _stream_ptr(stream):
 class SpmvBatch:
     """Many CSR matrices run as one persistent launch (optionally a replayed
     hipGraph).  items: list of dicts with keys m, n, row_ptr, col, val, x, y,
-    alpha, beta (tensors on one HIP device)."""
+    alpha, beta (tensors on one HIP device).  variant: 0 lets the autotuner
+    pick row blocks or column panels per matrix, 2 forces row blocks, 3 asks
+    for column panels wherever the format allows them."""
 
-    def __init__(self, items, subwave=0, chunks=0, item_size=0):
+    def __init__(self, items, subwave=0, chunks=0, item_size=0, variant=0, panel_rows=0):
         self._keep = []
         arr = (_lib.BatchItem * max(len(items), 1))()
         for i, it in enumerate(items):
@@ -79,7 +81,8 @@ class SpmvBatch:
             arr[i].d_y = it["y"].data_ptr()
             arr[i].alpha = float(it.get("alpha", 1.0))
             arr[i].beta = float(it.get("beta", 0.0))
-        o = make_opts(subwave=subwave, chunks=chunks, items=item_size)
+        o = make_opts(subwave=subwave, chunks=chunks, items=item_size, variant=variant,
+                      panel_rows=panel_rows)
         h = ctypes.c_void_p()
         check(lib().sblas_batch_create(len(items), arr, ctypes.byref(o), ctypes.byref(h)),
               "sblas_batch_create")

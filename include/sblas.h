@@ -269,12 +269,12 @@ typedef struct sblas_batch_item {
 
 /* What the autotuner picked for one matrix. */
 typedef struct sblas_batch_choice {
-  int kernel;      /* SBLAS_KERNEL_VECTOR (row-block form)                  */
+  int kernel;      /* SBLAS_KERNEL_VECTOR (kernel 1)                        */
   int subwave;     /* lanes per row                                          */
   int items;       /* elements per thread of a row block                     */
   int chunks;      /* nnz chunks the row blocks are cut from                 */
-  int row_blocks;  /* work items of this matrix                              */
-  int pad_;
+  int row_blocks;  /* work items of this matrix (row blocks or row panels)   */
+  int variant;     /* 2 row blocks, 3 column panels (x staged in LDS)        */
   double mean_row, cv_row;
   int64_t max_row;
 } sblas_batch_choice;
