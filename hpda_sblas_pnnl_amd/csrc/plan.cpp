@@ -404,7 +404,7 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
     launches++;
     if (tiles > 0 && merge_enabled) {
-      st = launch_tile_fixup(tiles, d_carry, Y, alpha, beta, s);
+      st = launch_tile_fixup(carry_count(), d_carry, Y, alpha, beta, s);
       launches++;
     }
     return st;
@@ -639,7 +639,8 @@ bool Plan::has_merge(int k) const {
 }
 
 int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) {
-  if (uses_tiles(k)) return tiles > 0 ? launch_tile_fixup(tiles, d_carry, Y, alpha, beta, s) : SBLAS_OK;
+  if (uses_tiles(k))
+    return tiles > 0 ? launch_tile_fixup(carry_count(), d_carry, Y, alpha, beta, s) : SBLAS_OK;
   if (variant == 3 && k == 1 && streams.empty())
     return pgroups > 0 ? launch_merge(d_pmrows, d_pmptr, d_pmslots, pgroups, panel.partials, 1, Y,
                                       alpha, beta, s)

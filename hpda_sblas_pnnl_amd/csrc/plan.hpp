@@ -104,6 +104,9 @@ struct Plan {
   bool merge_enabled = true;          // false: caller launches run_merge itself
   int tile_sub = 16;                  // lanes per row in the tile kernels
   bool uses_tiles(int k) const;
+  // carries left for the ordered fix-up: one per tile, or one per workgroup
+  // of four tiles for the CSR5-style kernel (its carries meet in LDS first)
+  int carry_count() const { return kernel == SBLAS_KERNEL_CSR5 ? (tiles + 3) / 4 : tiles; }
   bool has_merge(int k) const;
   int setup_tiles();
   int setup_csr5(const std::vector<int> &h_rp);

@@ -17,9 +17,11 @@
 //   * each lane sums its segments sequentially in FP64; rows that end inside
 //     one lane are written directly;
 //   * rows crossing lanes are finished by a segmented inclusive scan across
-//     the wave (wave shuffles), rows crossing tiles by the ordered carry
-//     fix-up shared with the merge-path and COO kernels.  alpha and beta are
-//     applied (the vendored CSR5 path ignores them, csr5_spmv_cuda.h:22,350)
+//     the wave on DPP lane moves (row_shr inside 16-lane rows, row_bcast
+//     across them); the four tiles of a workgroup stage their carries in LDS
+//     and fold them in order, and rows crossing workgroups are finished by
+//     the ordered carry fix-up shared with the merge-path and COO kernels.
+//     alpha and beta are applied (the vendored CSR5 path ignores them, csr5_spmv_cuda.h:22,350)
 //     and no floating-point atomics are used (it adds tile carries with
 //     atomicAdd, :350,377), so results do not depend on scheduling.
 #include "internal.hpp"
@@ -42,16 +44,69 @@ __device__ __forceinline__ double c5_axpby(double alpha, double s, double beta, 
   return (beta == 0.0) ? alpha * s : alpha * s + beta * yv;
 }
 
+// Wave-wide scans on DPP lane moves.  Lanes whose source lies outside the
+// move (row_shr past a 16-lane row start, rows that a broadcast does not
+// reach) read 0, the identity of both scans.
+//   row_shr:d   lane l reads l-d inside its row of 16 (d = 1, 2, 4, 8)
+//   row_bcast15 rows 1 and 3 read lane 15 of the row before (row mask 0xa)
+//   row_bcast31 rows 2 and 3 read lane 31 (row mask 0xc)
+// so after the four in-row steps each lane holds its row prefix, and the two
+// broadcasts add the totals of the rows before it.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_mov(double v) {
+  return __builtin_amdgcn_update_dpp(0.0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_mov(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+
+// inclusive prefix sum over the 64 lanes
+__device__ __forceinline__ int wave_incl_sum(int v) {
+  v += dpp_mov<kRowShr1, 0xf>(v);
+  v += dpp_mov<kRowShr2, 0xf>(v);
+  v += dpp_mov<kRowShr4, 0xf>(v);
+  v += dpp_mov<kRowShr8, 0xf>(v);
+  v += dpp_mov<kRowBcast15, 0xa>(v);
+  v += dpp_mov<kRowBcast31, 0xc>(v);
+  return v;
+}
+
+// One step of the segmented scan: (v, head) absorbs the moved-in partner
+// unless a segment already begins inside this lane's range.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step(double &v, int &head) {
+  const double vu = dpp_mov<CTRL, ROWS>(v);
+  const int hu = dpp_mov<CTRL, ROWS>(head);
+  if (!head) {
+    v += vu;
+    head = hu;
+  }
+}
+
+// inclusive segmented sum over the 64 lanes; head = 1 opens a segment
+__device__ __forceinline__ double wave_seg_sum(double v, int head) {
+  seg_step<kRowShr1, 0xf>(v, head);
+  seg_step<kRowShr2, 0xf>(v, head);
+  seg_step<kRowShr4, 0xf>(v, head);
+  seg_step<kRowShr8, 0xf>(v, head);
+  seg_step<kRowBcast15, 0xa>(v, head);
+  seg_step<kRowBcast31, 0xc>(v, head);
+  return v;
+}
+
+// One tile by one wave: rows closed inside the tile are written; the tile's
+// head / tail / spanning partials go to *out (lane 0).
 template <typename VT, int SIGMA>
-__global__ __launch_bounds__(256) void k_csr5(
-    int ntiles, int64_t nnz, const uint32_t *__restrict__ tile_nz,
+__device__ __forceinline__ void csr5_tile(
+    int p, int ntiles, int64_t nnz, const uint32_t *__restrict__ tile_nz,
     const uint32_t *__restrict__ flags, const int *__restrict__ rows_nz,
     const int *__restrict__ colT, const VT *__restrict__ valT, const double *__restrict__ x,
-    double *__restrict__ y, double alpha, double beta, TileCarry *__restrict__ carry) {
+    double *__restrict__ y, double alpha, double beta, TileCarry *out) {
   constexpr int T = 64 * SIGMA;
   const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= ntiles) return;  // uniform per wave
   const int64_t base = static_cast<int64_t>(p) * T;
   const uint32_t tnz = tile_nz[p];
   const int first_nz = static_cast<int>(tnz & 0x7fffffffu);
@@ -74,14 +129,9 @@ __global__ __launch_bounds__(256) void k_csr5(
 
   // rank of this lane's first row start among the tile's row starts
   const int cnt = __popc(f);
-  int incl = cnt;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int u = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += u;
-  }
+  const int incl = wave_incl_sum(cnt);
   const int excl = incl - cnt;
-  const int total_starts = __shfl(incl, 63, 64);
+  const int total_starts = __builtin_amdgcn_readlane(incl, 63);
 
   // in-lane segments
   const int64_t lane_first = base + static_cast<int64_t>(lane) * SIGMA;
@@ -106,21 +156,11 @@ __global__ __launch_bounds__(256) void k_csr5(
   const double a = has_start ? first : cur;      // part before this lane's first start
   const double last = has_start ? cur : 0.0;     // part after its last start
 
-  // segmented inclusive scan of a; a segment begins after each lane with a start
-  double sv = a;
-  // every lane takes part in each shuffle; the select comes after
-  const int prev_has = __shfl_up(has_start ? 1 : 0, 1, 64);
-  int head = (lane == 0) ? 1 : prev_has;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double vu = __shfl_up(sv, d, 64);
-    const int hu = __shfl_up(head, d, 64);
-    if (lane >= d && !head) {
-      sv += vu;
-      head = hu;
-    }
-  }
+  // segmented inclusive scan of a; a segment begins after each lane with a
+  // start (lane 0 always begins one)
   const unsigned long long starts_mask = __ballot(has_start);
+  const int head = (lane == 0) ? 1 : static_cast<int>((starts_mask >> (lane - 1)) & 1ull);
+  const double sv = wave_seg_sum(a, head);
   const unsigned long long below = starts_mask & ((1ull << lane) - 1ull);
   const int prev = below ? 63 - __clzll(static_cast<long long>(below)) : -1;
   const double prev_last = __shfl(last, prev < 0 ? 0 : prev, 64);
@@ -141,7 +181,7 @@ __global__ __launch_bounds__(256) void k_csr5(
   // the segment open at the tile's end
   const unsigned long long any = starts_mask;
   const int last_lane = any ? 63 - __clzll(static_cast<long long>(any)) : -1;
-  const double tail_scan = __shfl(sv, 63, 64);
+  const double tail_scan = __shfl(sv, 63, 64);  // uniform source lane
   const double open_last = __shfl(last, last_lane < 0 ? 0 : last_lane, 64);
   // the head partial (if any) sits in the lowest lane with a start
   const int first_lane = any ? __ffsll(static_cast<long long>(any)) - 1 : 0;
@@ -175,8 +215,69 @@ __global__ __launch_bounds__(256) void k_csr5(
         cy.tail_row = open_row;
       }
     }
-    carry[p] = cy;
+    *out = cy;
   }
+}
+
+// Four tiles per workgroup.  Their carries are staged in LDS and folded in
+// tile order by one thread: rows that begin and end inside the workgroup are
+// written here, and one carry per workgroup (same meaning as a tile's) is
+// left for the ordered fix-up over workgroups.
+template <typename VT, int SIGMA>
+__global__ __launch_bounds__(256) void k_csr5(
+    int ntiles, int64_t nnz, const uint32_t *__restrict__ tile_nz,
+    const uint32_t *__restrict__ flags, const int *__restrict__ rows_nz,
+    const int *__restrict__ colT, const VT *__restrict__ valT, const double *__restrict__ x,
+    double *__restrict__ y, double alpha, double beta, TileCarry *__restrict__ carry) {
+  __shared__ TileCarry wc[4];
+  const int wave = threadIdx.x >> 6;
+  const int p = blockIdx.x * 4 + wave;
+  if (p < ntiles)  // uniform per wave; every wave reaches the barrier below
+    csr5_tile<VT, SIGMA>(p, ntiles, nnz, tile_nz, flags, rows_nz, colT, valT, x, y, alpha, beta,
+                         &wc[wave]);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int count = min(4, ntiles - static_cast<int>(blockIdx.x) * 4);
+  TileCarry out{0.0, 0.0, -1, -1, 0, 0};
+  bool all_span = true;
+  int open = -1;  // row opened by a tile of this workgroup and not yet closed
+  double osum = 0.0;
+  for (int i = 0; i < count; i++) {
+    const TileCarry c = wc[i];
+    if (c.spanning) {
+      if (open >= 0) {
+        osum += c.head;
+      } else {
+        out.head += c.head;
+        out.head_row = c.head_row;
+      }
+      continue;
+    }
+    all_span = false;
+    if (c.head_row >= 0) {
+      if (open >= 0) {
+        y[open] = c5_axpby(alpha, osum + c.head, beta, y[open]);
+        open = -1;
+      } else {
+        out.head += c.head;
+        out.head_row = c.head_row;
+      }
+    } else if (open >= 0) {
+      y[open] = c5_axpby(alpha, osum, beta, y[open]);
+      open = -1;
+    }
+    if (c.tail_row >= 0) {
+      open = c.tail_row;
+      osum = c.tail;
+    }
+  }
+  if (all_span) {
+    out.spanning = 1;
+  } else if (open >= 0) {
+    out.tail = osum;
+    out.tail_row = open;
+  }
+  carry[blockIdx.x] = out;
 }
 
 // Transposed, zero-padded copy: element (lane l, s) of tile p moves to
