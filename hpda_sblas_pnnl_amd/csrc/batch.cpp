@@ -57,6 +57,7 @@ struct sblas_batch_s {
   sblas::BatchItem *d_items = nullptr;
   double *d_partials = nullptr;
   int *d_ctr = nullptr;
+  int *d_qbound = nullptr;
   int *d_gmat = nullptr, *d_grow = nullptr, *d_gptr = nullptr, *d_gslot = nullptr;
   int groups = 0;
   int grid = 0;
@@ -92,7 +93,7 @@ struct sblas_batch_s {
   }
   int enqueue(hipStream_t s) {
     int st = sblas::launch_batch(items_per_thread, grid, static_cast<int>(items.size()), d_items,
-                                 d_mats, d_partials, d_ctr, s);
+                                 d_mats, d_partials, d_qbound, d_ctr, s);
     if (st != SBLAS_OK) return st;
     return sblas::launch_batch_merge(groups, d_gmat, d_grow, d_gptr, d_gslot, d_mats, d_partials,
                                      s);
@@ -120,7 +121,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     return code;
   };
   std::vector<int> gmat, grow, gptr(1, 0), gslot;
-  // work items of each matrix in its own order, before the list is sorted
+  // work items of each matrix in its own order
   struct Pending {
     BatchItem item;
     int64_t work;        // elements
@@ -240,22 +241,29 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
                                             rs.cv, rs.max_row});
     h_rps[i] = std::move(h_rp);
   }
-  // list order: row panels heaviest first, then row blocks in matrix order
+  // list order: matrix by matrix, each matrix's items in its own order (row
+  // panels of one matrix stay neighbours, so they share x in one XCD's L2)
   std::vector<std::pair<int, int>> order;  // (matrix, index in its list)
   for (int i = 0; i < count; i++)
-    for (int q = 0; q < static_cast<int>(pending[i].size()); q++)
-      if (pending[i][q].item.kind == kBatchPanel) order.emplace_back(i, q);
-  std::stable_sort(order.begin(), order.end(), [&](const auto &a, const auto &c2) {
-    return pending[a.first][a.second].work > pending[c2.first][c2.second].work;
-  });
-  for (int i = 0; i < count; i++)
-    for (int q = 0; q < static_cast<int>(pending[i].size()); q++)
-      if (pending[i][q].item.kind == kBatchRowBlock) order.emplace_back(i, q);
+    for (int q = 0; q < static_cast<int>(pending[i].size()); q++) order.emplace_back(i, q);
   std::vector<std::vector<int>> where(count);
   for (int i = 0; i < count; i++) where[i].assign(pending[i].size(), -1);
+  int64_t total_work = 0;
   for (const auto &o : order) {
     where[o.first][o.second] = static_cast<int>(b->items.size());
     b->items.push_back(pending[o.first][o.second].item);
+    total_work += pending[o.first][o.second].work + 256;  // elements + per-item cost
+  }
+  // eight queues of about equal work, contiguous in the list
+  std::vector<int> qbound(kBatchQueues + 1, static_cast<int>(b->items.size()));
+  qbound[0] = 0;
+  {
+    int64_t run = 0;
+    int q = 1;
+    for (size_t w = 0; w < order.size() && q < kBatchQueues; w++) {
+      while (q < kBatchQueues && run >= total_work * q / kBatchQueues) qbound[q++] = static_cast<int>(w);
+      run += pending[order[w].first][order[w].second].work + 256;
+    }
   }
   // rows cut by an item's element window: partial slots 2w (first row) and
   // 2w+1 (last row), listed per row in the matrix's own item order
@@ -293,7 +301,9 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     return fail(st);
   std::vector<double> zeros(2 * std::max<size_t>(b->items.size(), 1), 0.0);
   if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);
-  if ((st = b->upload(&b->d_ctr, std::vector<int>{0, 0})) != SBLAS_OK) return fail(st);
+  if ((st = b->upload(&b->d_ctr, std::vector<int>(kBatchQueues + 1, 0))) != SBLAS_OK ||
+      (st = b->upload(&b->d_qbound, qbound)) != SBLAS_OK)
+    return fail(st);
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

@@ -125,6 +125,7 @@ int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha
 // column-panel form of kernel 1 (sblas_panel.hip)
 constexpr int kPanelCols = 4096;  // x panel staged in LDS (32 KB of FP64)
 constexpr int kPanelRowsMax = 2048;  // rows per row panel (8 per thread)
+constexpr int kPanelPad = 8;         // slices start on 8-element boundaries
 // One row panel: rows [row0, row0 + nrow) of one execution chunk, its first
 // column panel, and flags: bit 0 = the first row is cut by the chunk's
 // element window (partial slot 2p), bit 1 = the last row is (slot 2p+1).
@@ -145,9 +146,10 @@ struct PanelFormat {
   // holds at most 65535 elements, else 32-bit (srp32)
   uint16_t *srp = nullptr;
   uint32_t *srp32 = nullptr;
-  void *pval = nullptr;         // values, slice-major
-  uint16_t *pcol = nullptr;     // column offsets inside the column panel
+  void *pval = nullptr;         // values, slice-major, slices padded to kPanelPad
+  uint16_t *pcol = nullptr;     // column offsets inside the column panel (padding: 0)
   double *partials = nullptr;   // 2 slots per row panel
+  int64_t padded = 0;           // elements stored, slices padded
   std::vector<ExecChunk> hpanels;  // host copy: element window and rows of each panel
   std::vector<void *> buffers;
 };
@@ -195,12 +197,15 @@ struct BatchItem {  // one row block, or one row panel, of one matrix
   int lo, hi;              // row block: element window
   int pad_[2];
 };
-// Persistent launch over all items: workgroups take items from a counter in
-// ctr[0] (ctr[1] counts finished workgroups; the last one resets both), so
-// heavy items listed first spread over the GPU.  Each item's arithmetic does
-// not depend on which workgroup runs it.
+// Persistent launch over all items: the list is cut into kBatchQueues
+// contiguous queues [qbound[q], qbound[q+1]); workgroups take items from
+// per-queue counters ctr[q] (ctr[kBatchQueues] counts finished workgroups;
+// the last one resets them all).  Each item's arithmetic does not depend on
+// which workgroup runs it.
+constexpr int kBatchQueues = 8;  // one per XCD
 int launch_batch(int items_per_thread, int grid, int nitems, const BatchItem *items,
-                 const BatchMatrix *mats, double *partials, int *ctr, hipStream_t s);
+                 const BatchMatrix *mats, double *partials, const int *qbound, int *ctr,
+                 hipStream_t s);
 // Resident workgroups per CU of the batched kernel.
 int batch_occupancy(int items_per_thread);
 int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,

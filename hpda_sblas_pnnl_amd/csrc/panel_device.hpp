@@ -9,17 +9,6 @@
 namespace sblas {
 namespace panel {
 
-__device__ __forceinline__ double pv(const double *p, int64_t j) {
-  return __builtin_nontemporal_load(p + j);
-}
-__device__ __forceinline__ double pv(const float *p, int64_t j) {
-  return static_cast<double>(__builtin_nontemporal_load(p + j));
-}
-__device__ __forceinline__ double pv(const uint16_t *p, int64_t j) {
-  const uint32_t h = __builtin_nontemporal_load(p + j);
-  return static_cast<double>(__uint_as_float(h << 16));
-}
-
 // Separately rounded multiply and add.  Contraction is switched off inside
 // these bodies, so a product never fuses with the add that consumes it (the
 // header intrinsics are plain * and + and would fuse under -ffp-contract=fast).
@@ -32,11 +21,57 @@ __device__ __forceinline__ double add_rn(double a, double b) {
   return a + b;
 }
 
-constexpr int kItems = 8;               // elements per thread per chunk
+constexpr int kItems = 8;               // consecutive elements per thread per chunk
 constexpr int kChunk = 256 * kItems;    // elements per chunk of a slice
 constexpr int kLdsDoubles = kPanelCols + kChunk;  // x panel + products
+static_assert(kItems == kPanelPad, "a thread's elements never straddle a slice end");
 
-// Current slice of a row panel and the first element of the chunk worked on.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Eight consecutive values from a 64-byte (FP64), 32-byte (FP32) or 16-byte
+// (BF16) aligned address, as doubles, with 16-byte non-temporal loads.
+__device__ __forceinline__ void load8(const double *p, double *v) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const f64x2 q = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p) + i);
+    v[2 * i] = q.x;
+    v[2 * i + 1] = q.y;
+  }
+}
+__device__ __forceinline__ void load8(const float *p, double *v) {
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const f32x4 q = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p) + i);
+    v[4 * i] = q.x;
+    v[4 * i + 1] = q.y;
+    v[4 * i + 2] = q.z;
+    v[4 * i + 3] = q.w;
+  }
+}
+__device__ __forceinline__ void load8(const uint16_t *p, double *v) {
+  const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    v[2 * i] = static_cast<double>(__uint_as_float(w[i] << 16));
+    v[2 * i + 1] = static_cast<double>(__uint_as_float(w[i] & 0xffff0000u));
+  }
+}
+// eight consecutive 16-bit column offsets from a 16-byte aligned address
+__device__ __forceinline__ void load8_cols(const uint16_t *p, int *l) {
+  const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    l[2 * i] = static_cast<int>(w[i] & 0xffffu);
+    l[2 * i + 1] = static_cast<int>(w[i] >> 16);
+  }
+}
+
+// Current slice of a row panel and the first element of the chunk worked on;
+// len is the slice's length padded to a multiple of kPanelPad.
 struct Cursor {
   int s, k0, len;
   int64_t base;
@@ -104,15 +139,13 @@ __device__ __forceinline__ void row_panel(
       e[j] = in ? hi : 0;
     }
   };
-  // values and column offsets of a chunk: consecutive threads on consecutive
-  // elements (whole lines per load instruction), all issued back to back
+  // values and column offsets of a chunk: thread t takes kItems consecutive
+  // elements (a wave reads one contiguous run), with aligned 16-byte loads;
+  // threads past the slice's end re-read its last group (never summed)
   auto load_chunk = [&](const Cursor &c) {
-#pragma unroll
-    for (int i = 0; i < kItems; i++) {
-      const int64_t k = c.base + min(c.k0 + t + i * 256, c.len - 1);
-      v[i] = pv(pval, k);
-      l[i] = __builtin_nontemporal_load(pcol + k);
-    }
+    const int64_t k = c.base + min(c.k0 + t * kItems, c.len - kItems);
+    load8(pval + k, v);
+    load8_cols(pcol + k, l);
   };
 
   Cursor cur;
@@ -126,7 +159,7 @@ __device__ __forceinline__ void row_panel(
   while (cur.s < s1) {
     // products of the current chunk (x of its slice is in LDS)
 #pragma unroll
-    for (int i = 0; i < kItems; i++) prod[t + i * 256] = mul_rn(v[i], xs[l[i]]);
+    for (int i = 0; i < kItems; i++) prod[t * kItems + i] = mul_rn(v[i], xs[l[i]]);
     Cursor nxt = cur;
     nxt.k0 += kChunk;
     if (nxt.k0 >= cur.len) seek(nxt, cur.s + 1, s1, slice_off);
@@ -135,12 +168,19 @@ __device__ __forceinline__ void row_panel(
     __syncthreads();  // products visible; x of the current slice no longer read
     const bool new_slice = nxt.s != cur.s && nxt.s < s1;
     if (new_slice) stage_x(nxt.s);
-    // each row adds its products of this chunk in element order
+    // each row adds its products of this chunk in element order; eight LDS
+    // reads (clamped inside the row) are in flight before the ordered adds
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
-      const int lo = max(b[j], cur.k0), hi = min(e[j], cur.k0 + kChunk);
+      const int lo = max(b[j], cur.k0) - cur.k0, hi = min(e[j], cur.k0 + kChunk) - cur.k0;
       double a = acc[j];
-      for (int k = lo; k < hi; k++) a = add_rn(a, prod[k - cur.k0]);
+      for (int k = lo; k < hi; k += 8) {
+        double q[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) q[i] = prod[min(k + i, hi - 1)];
+#pragma unroll
+        for (int i = 0; i < 8; i++) a = (k + i < hi) ? add_rn(a, q[i]) : a;
+      }
       acc[j] = a;
     }
     if (new_slice) row_ranges(nxt.s);

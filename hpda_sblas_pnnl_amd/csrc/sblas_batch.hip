@@ -3,9 +3,11 @@
 // Every matrix is cut into work items of the form its autotuner choice
 // names: row panels of the column-panel form (matrices whose rows draw x from
 // a wide window) or row blocks of the LDS-staged row form, and all items of
-// all matrices form one list, heaviest first.  A persistent grid sized to
-// the resident workgroups takes items from a device counter, so the launch
-// balances itself across the GPU; an item's arithmetic (and so every sum) is
+// all matrices form one list in matrix order, cut into eight queues of equal
+// work, one per XCD.  A persistent grid sized to the resident workgroups
+// takes items from per-queue device counters (its own XCD's queue first,
+// then the others), so the launch balances itself across the GPU while
+// neighbouring items share that XCD's L2; an item's arithmetic (and so every sum) is
 // the same whichever workgroup runs it.  Each item carries its matrix's
 // parameters (arrays, value type, lanes per row), so matrices with different
 // profiles share the launch.  Rows cut by an item's element window leave
@@ -144,6 +146,7 @@ template <int ITEMS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__restrict__ items,
                                                  const BatchMatrix *__restrict__ mats,
                                                  double *__restrict__ partials,
+                                                 const int *__restrict__ qbound,
                                                  int *__restrict__ ctr) {
   static_assert(BLOCK == 256, "the panel body assumes 256 threads");
   constexpr int LDS_RB = BLOCK * ITEMS + kStreamRowsMax;
@@ -152,27 +155,35 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
   __shared__ double wsum[BLOCK / 64];
   __shared__ int next;
   const int t = threadIdx.x;
-  for (;;) {
-    if (t == 0) next = atomicAdd(&ctr[0], 1);
-    __syncthreads();
-    const int w = next;
-    __syncthreads();  // every thread has read `next` before it is taken again
-    if (w >= nitems) break;  // the counter only grows: every workgroup gets here
-    const BatchItem it = items[w];
-    const BatchMatrix &M = mats[it.mat];
-    if (it.kind == kBatchPanel)
-      row_panel_any(w, it, M, partials, lds);
-    else
-      row_block<ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+  // workgroups are dispatched to the eight XCDs in turn: start on this XCD's
+  // queue (a contiguous run of the list, so neighbouring row panels share x
+  // in that XCD's L2), then help the other queues in order
+  const int home = static_cast<int>(blockIdx.x) & (kBatchQueues - 1);
+  for (int d = 0; d < kBatchQueues; d++) {
+    const int q = (home + d) & (kBatchQueues - 1);
+    const int end = qbound[q + 1];
+    for (;;) {
+      if (t == 0) next = qbound[q] + atomicAdd(&ctr[q], 1);
+      __syncthreads();
+      const int w = next;
+      __syncthreads();  // every thread has read `next` before it is taken again
+      if (w >= end) break;  // counters only grow: every workgroup gets here
+      const BatchItem it = items[w];
+      const BatchMatrix &M = mats[it.mat];
+      if (it.kind == kBatchPanel)
+        row_panel_any(w, it, M, partials, lds);
+      else
+        row_block<ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+    }
   }
   if (t == 0) {
     // the last workgroup to finish rewinds the counters for the next launch
-    const int done = atomicAdd(&ctr[1], 1);
+    const int done = atomicAdd(&ctr[kBatchQueues], 1);
     if (done == static_cast<int>(gridDim.x) - 1) {
-      atomicExch(&ctr[0], 0);
-      atomicExch(&ctr[1], 0);
+      for (int q = 0; q <= kBatchQueues; q++) atomicExch(&ctr[q], 0);
     }
   }
+  (void)nitems;
 }
 
 // Split rows of all matrices: one thread per row, partials in block order.
@@ -192,15 +203,16 @@ __global__ void k_batch_merge(int groups, const int *__restrict__ gmat,
 }  // namespace
 
 int launch_batch(int items_per_thread, int grid, int nitems, const BatchItem *items,
-                 const BatchMatrix *mats, double *partials, int *ctr, hipStream_t s) {
+                 const BatchMatrix *mats, double *partials, const int *qbound, int *ctr,
+                 hipStream_t s) {
   if (nitems <= 0 || grid <= 0) return SBLAS_OK;
   constexpr int B = kStreamBlock;
   if (items_per_thread == 16)
     hipLaunchKernelGGL((k_batch<16, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats, partials,
-                       ctr);
+                       qbound, ctr);
   else
     hipLaunchKernelGGL((k_batch<8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats, partials,
-                       ctr);
+                       qbound, ctr);
   SBLAS_HIP_TRY(hipGetLastError());
   return SBLAS_OK;
 }

@@ -8,8 +8,9 @@
 // columns.  A workgroup owns one row panel and walks its column panels in
 // increasing order: it stages x[q*W, (q+1)*W) in LDS with coalesced loads
 // (one pass over the panel's x lines), then streams that slice's elements in
-// chunks of 2048 with consecutive threads on consecutive elements, multiplies
-// each by its x from LDS and leaves the products in LDS.  The plan stores each
+// chunks of 2048 (each thread eight consecutive elements, read with aligned
+// 16-byte loads: slices are padded to 8 elements), multiplies each by its x
+// from LDS and leaves the products in LDS.  The plan stores each
 // (row panel, column panel) slice contiguously, rows in order, with 16-bit
 // column offsets inside the panel and row offsets inside the slice (16-bit
 // when every slice holds at most 65535 elements, else 32-bit).
@@ -163,7 +164,7 @@ __global__ void k_panel_slice_scan(int nslices, int R, const uint32_t *__restric
   }
   if (threadIdx.x == 0) {
     srp[static_cast<int64_t>(s) * (R + 1) + R] = total;
-    snnz[s] = total;
+    snnz[s] = (total + kPanelPad - 1) / kPanelPad * kPanelPad;  // slices start aligned
     if (total > 65535u) overflow[0] = 1;
   }
 }
@@ -307,12 +308,8 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   int64_t *snnz = static_cast<int64_t *>(alloc(sizeof(int64_t) * (ns + 1)));
   f.slice_off = static_cast<int64_t *>(alloc(sizeof(int64_t) * (ns + 1)));
   int *overflow = static_cast<int *>(alloc(sizeof(int)));
-  const size_t vbytes = val_type == SBLAS_VAL_F64 ? 8 : (val_type == SBLAS_VAL_F32 ? 4 : 2);
-  f.pval = alloc(vbytes * std::max<int64_t>(nnz, 1));
-  f.pcol = static_cast<uint16_t *>(alloc(sizeof(uint16_t) * std::max<int64_t>(nnz, 1)));
   f.partials = static_cast<double *>(alloc(sizeof(double) * 2 * np));
-  if (!cnt || !f.srp32 || !snnz || !f.slice_off || !overflow || !f.pval || !f.pcol ||
-      !f.partials) {
+  if (!cnt || !f.srp32 || !snnz || !f.slice_off || !overflow || !f.partials) {
     set_error(SBLAS_ERR_ALLOC, "panel format: allocation failed");
     return SBLAS_ERR_ALLOC;
   }
@@ -367,6 +364,32 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{2},
                        reinterpret_cast<uint32_t *>(snnz + f.nslices));
     SBLAS_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, snnz, f.slice_off, f.nslices + 1, s));
+  }
+  // element arrays: slices padded to kPanelPad elements, padding zeroed
+  int64_t padded = 0;
+  if (f.nslices > 0) {
+    SBLAS_HIP_TRY(hipMemcpyAsync(&padded, f.slice_off + f.nslices, sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, s));
+    SBLAS_HIP_TRY(hipStreamSynchronize(s));
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{2},
+                       reinterpret_cast<uint32_t *>(f.slice_off));
+  }
+  f.padded = padded;
+  const size_t vbytes = val_type == SBLAS_VAL_F64 ? 8 : (val_type == SBLAS_VAL_F32 ? 4 : 2);
+  const int64_t cap = std::max<int64_t>(padded, kPanelPad);
+  f.pval = alloc(vbytes * cap);
+  f.pcol = static_cast<uint16_t *>(alloc(sizeof(uint16_t) * cap));
+  if (!f.pval || !f.pcol) {
+    set_error(SBLAS_ERR_ALLOC, "panel format: allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  {
+    const int64_t wv = static_cast<int64_t>(vbytes) * cap / 4, wc = cap * 2 / 4;
+    hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((wv + 255) / 256)), dim3(256), 0, s,
+                       wv, static_cast<uint32_t *>(f.pval));
+    hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((wc + 255) / 256)), dim3(256), 0, s,
+                       wc, reinterpret_cast<uint32_t *>(f.pcol));
   }
   if (f.npanels > 0) {
     const dim3 g(f.npanels), b(256);
