@@ -133,7 +133,7 @@ SIGNATURES = {
 
 class BatchItem(ctypes.Structure):
     _fields_ = [("m", _I), ("n", _I), ("nnz", _I64), ("d_row_ptr", _P), ("d_col", _P),
-                ("d_val", _P), ("val_type", _I), ("pad_", _I), ("d_x", _P), ("d_y", _P),
+                ("d_val", _P), ("val_type", _I), ("k", _I), ("d_x", _P), ("d_y", _P),
                 ("alpha", _D), ("beta", _D)]
 
 

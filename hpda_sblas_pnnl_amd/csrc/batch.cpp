@@ -134,10 +134,12 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     const sblas_batch_item &it = items[i];
     if (it.m < 0 || it.n < 0 || it.nnz < 0 || it.nnz > INT32_MAX - 4096 || !it.d_row_ptr ||
         (it.nnz > 0 && (!it.d_col || !it.d_val || !it.d_x)) || (it.m > 0 && !it.d_y) ||
-        it.val_type < SBLAS_VAL_F64 || it.val_type > SBLAS_VAL_BF16) {
+        it.val_type < SBLAS_VAL_F64 || it.val_type > SBLAS_VAL_BF16 ||
+        !(it.k == 0 || it.k == 1 || it.k == 2 || it.k == 4 || it.k == 8 || it.k == 16)) {
       set_error(SBLAS_ERR_ARGS, "batch: bad item " + std::to_string(i));
       return fail(SBLAS_ERR_ARGS);
     }
+    const int kv = it.k <= 1 ? 1 : it.k;
     std::vector<int> h_rp(static_cast<size_t>(it.m) + 1);
     if (hipMemcpy(h_rp.data(), it.d_row_ptr, sizeof(int) * h_rp.size(), hipMemcpyDeviceToHost) !=
         hipSuccess) {
@@ -179,6 +181,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     M.subwave = c.subwave;
     M.m = it.m;
     M.n = it.n;
+    M.k = kv;
 
     // column-panel form when the matrix's column structure pays for it (one
     // chunk: rows are never cut; 16-bit slice offsets only)
@@ -186,7 +189,8 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     b->panels.emplace_back();
     PanelFormat &f = b->panels.back();
     bool panel = false;
-    if (it.m > 0 && it.nnz > 0 && want != 1 && want != 2 && (want == 3 || rs.max_row <= 8192)) {
+    if (kv == 1 && it.m > 0 && it.nnz > 0 && want != 1 && want != 2 &&
+        (want == 3 || rs.max_row <= 8192)) {
       const std::vector<ExecChunk> one{ExecChunk{0, it.nnz, 0, it.m}};
       const int rows = (opts && opts->panel_rows > 0) ? opts->panel_rows : 1024;
       int st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col,
@@ -223,7 +227,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     if (panel) {
       for (size_t q = 0; q < f.hpanels.size(); q++) {
         const ExecChunk &e = f.hpanels[q];
-        mine.push_back(Pending{BatchItem{i, kBatchPanel, static_cast<int>(q), 0, 0, 0, {0, 0}},
+        mine.push_back(Pending{BatchItem{i, kBatchPanel, static_cast<int>(q), 0, 0, 0, 0, 0},
                                e.hi - e.lo, static_cast<int>(e.lo), static_cast<int>(e.hi),
                                e.row_begin, e.row_end});
       }
@@ -233,7 +237,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       const std::vector<ExecChunk> ex = exec_chunks_from_bounds(it.m, h_rp.data(), bounds);
       for (const RowBlock &r : build_row_blocks(ex, h_rp.data(), kStreamBlock * c.items))
         mine.push_back(Pending{BatchItem{i, kBatchRowBlock, r.row_begin, r.row_end, r.lo, r.hi,
-                                         {0, 0}},
+                                         0, 0},
                                r.hi - r.lo, r.lo, r.hi, r.row_begin, r.row_end});
     }
     b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks,
@@ -249,10 +253,19 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
   std::vector<std::vector<int>> where(count);
   for (int i = 0; i < count; i++) where[i].assign(pending[i].size(), -1);
   int64_t total_work = 0;
+  int64_t nslots = 0;
   for (const auto &o : order) {
     where[o.first][o.second] = static_cast<int>(b->items.size());
-    b->items.push_back(pending[o.first][o.second].item);
-    total_work += pending[o.first][o.second].work + 256;  // elements + per-item cost
+    BatchItem item = pending[o.first][o.second].item;
+    const int kv = b->mats[o.first].k;
+    item.pslot = static_cast<int>(nslots);  // 2 slots of kv partials each
+    nslots += 2 * kv;
+    b->items.push_back(item);
+    total_work += pending[o.first][o.second].work * kv + 256;  // elements + per-item cost
+  }
+  if (nslots > INT32_MAX) {
+    set_error(SBLAS_ERR_ARGS, "batch: too many work items");
+    return fail(SBLAS_ERR_ARGS);
   }
   // eight queues of about equal work, contiguous in the list
   std::vector<int> qbound(kBatchQueues + 1, static_cast<int>(b->items.size()));
@@ -262,11 +275,11 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     int q = 1;
     for (size_t w = 0; w < order.size() && q < kBatchQueues; w++) {
       while (q < kBatchQueues && run >= total_work * q / kBatchQueues) qbound[q++] = static_cast<int>(w);
-      run += pending[order[w].first][order[w].second].work + 256;
+      run += pending[order[w].first][order[w].second].work * b->mats[order[w].first].k + 256;
     }
   }
-  // rows cut by an item's element window: partial slots 2w (first row) and
-  // 2w+1 (last row), listed per row in the matrix's own item order
+  // rows cut by an item's element window: partial slots pslot (first row) and
+  // pslot + k (last row), listed per row in the matrix's own item order
   for (int i = 0; i < count; i++) {
     const std::vector<int> &h_rp = h_rps[i];
     for (size_t q = 0; q < pending[i].size(); q++) {
@@ -286,8 +299,9 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       };
       const int first = r.row_begin, last = r.row_end - 1;
       if (first > last) continue;
-      if (split(first)) add(first, 2 * w);
-      if (last != first && split(last)) add(last, 2 * w + 1);
+      const int base = b->items[w].pslot, kv = b->mats[i].k;
+      if (split(first)) add(first, base);
+      if (last != first && split(last)) add(last, base + kv);
     }
   }
   b->groups = static_cast<int>(grow.size());
@@ -299,7 +313,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       (st = b->upload(&b->d_gptr, gptr)) != SBLAS_OK ||
       (st = b->upload(&b->d_gslot, gslot)) != SBLAS_OK)
     return fail(st);
-  std::vector<double> zeros(2 * std::max<size_t>(b->items.size(), 1), 0.0);
+  std::vector<double> zeros(static_cast<size_t>(std::max<int64_t>(nslots, 1)), 0.0);
   if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);
   if ((st = b->upload(&b->d_ctr, std::vector<int>(kBatchQueues + 1, 0))) != SBLAS_OK ||
       (st = b->upload(&b->d_qbound, qbound)) != SBLAS_OK)

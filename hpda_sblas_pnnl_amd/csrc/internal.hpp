@@ -186,7 +186,7 @@ struct BatchMatrix {
   const void *pval;
   const uint16_t *pcol;
   int prows;    // rows per row panel
-  int pad_;
+  int k;        // right-hand sides (x, y are n x k, m x k row-major); 1 = one vector
 };
 constexpr int kBatchRowBlock = 0;
 constexpr int kBatchPanel = 1;
@@ -195,7 +195,8 @@ struct BatchItem {  // one row block, or one row panel, of one matrix
   int kind;     // kBatchRowBlock or kBatchPanel
   int row_begin, row_end;  // row block: rows; row panel: row_begin = panel index
   int lo, hi;              // row block: element window
-  int pad_[2];
+  int pslot;               // first partial slot: 2*k slots (first row, last row)
+  int pad_;
 };
 // Persistent launch over all items: the list is cut into kBatchQueues
 // contiguous queues [qbound[q], qbound[q+1]); workgroups take items from

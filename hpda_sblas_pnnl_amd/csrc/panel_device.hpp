@@ -21,53 +21,35 @@ __device__ __forceinline__ double add_rn(double a, double b) {
   return a + b;
 }
 
-constexpr int kItems = 8;               // consecutive elements per thread per chunk
+constexpr int kItems = 8;               // elements per thread per chunk (4 pairs)
 constexpr int kChunk = 256 * kItems;    // elements per chunk of a slice
 constexpr int kLdsDoubles = kPanelCols + kChunk;  // x panel + products
-static_assert(kItems == kPanelPad, "a thread's elements never straddle a slice end");
+static_assert(kPanelPad % 2 == 0, "pairs never straddle a slice end");
 
 typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-// Eight consecutive values from a 64-byte (FP64), 32-byte (FP32) or 16-byte
-// (BF16) aligned address, as doubles, with 16-byte non-temporal loads.
-__device__ __forceinline__ void load8(const double *p, double *v) {
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const f64x2 q = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p) + i);
-    v[2 * i] = q.x;
-    v[2 * i + 1] = q.y;
-  }
+// Two consecutive values (an aligned pair) as doubles, one non-temporal load.
+__device__ __forceinline__ void load2(const double *p, double &a, double &b) {
+  const f64x2 q = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p));
+  a = q.x;
+  b = q.y;
 }
-__device__ __forceinline__ void load8(const float *p, double *v) {
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    const f32x4 q = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p) + i);
-    v[4 * i] = q.x;
-    v[4 * i + 1] = q.y;
-    v[4 * i + 2] = q.z;
-    v[4 * i + 3] = q.w;
-  }
+__device__ __forceinline__ void load2(const float *p, double &a, double &b) {
+  const f32x2 q = __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(p));
+  a = q.x;
+  b = q.y;
 }
-__device__ __forceinline__ void load8(const uint16_t *p, double *v) {
-  const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
-  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    v[2 * i] = static_cast<double>(__uint_as_float(w[i] << 16));
-    v[2 * i + 1] = static_cast<double>(__uint_as_float(w[i] & 0xffff0000u));
-  }
+__device__ __forceinline__ void load2(const uint16_t *p, double &a, double &b) {
+  const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+  a = static_cast<double>(__uint_as_float(w << 16));
+  b = static_cast<double>(__uint_as_float(w & 0xffff0000u));
 }
-// eight consecutive 16-bit column offsets from a 16-byte aligned address
-__device__ __forceinline__ void load8_cols(const uint16_t *p, int *l) {
-  const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
-  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    l[2 * i] = static_cast<int>(w[i] & 0xffffu);
-    l[2 * i + 1] = static_cast<int>(w[i] >> 16);
-  }
+// two consecutive 16-bit column offsets
+__device__ __forceinline__ void load2_cols(const uint16_t *p, int &a, int &b) {
+  const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+  a = static_cast<int>(w & 0xffffu);
+  b = static_cast<int>(w >> 16);
 }
 
 // Current slice of a row panel and the first element of the chunk worked on;
@@ -139,13 +121,17 @@ __device__ __forceinline__ void row_panel(
       e[j] = in ? hi : 0;
     }
   };
-  // values and column offsets of a chunk: thread t takes kItems consecutive
-  // elements (a wave reads one contiguous run), with aligned 16-byte loads;
-  // threads past the slice's end re-read its last group (never summed)
+  // values and column offsets of a chunk: pair i of thread t is elements
+  // 512*i + 2t, +1 (so one load instruction of a wave is a contiguous run of
+  // whole lines), all issued back to back; threads past the slice's end
+  // re-read its last pair (never summed)
   auto load_chunk = [&](const Cursor &c) {
-    const int64_t k = c.base + min(c.k0 + t * kItems, c.len - kItems);
-    load8(pval + k, v);
-    load8_cols(pcol + k, l);
+#pragma unroll
+    for (int i = 0; i < kItems / 2; i++) {
+      const int64_t k = c.base + min(c.k0 + 512 * i + 2 * t, c.len - 2);
+      load2(pval + k, v[2 * i], v[2 * i + 1]);
+      load2_cols(pcol + k, l[2 * i], l[2 * i + 1]);
+    }
   };
 
   Cursor cur;
@@ -159,7 +145,12 @@ __device__ __forceinline__ void row_panel(
   while (cur.s < s1) {
     // products of the current chunk (x of its slice is in LDS)
 #pragma unroll
-    for (int i = 0; i < kItems; i++) prod[t * kItems + i] = mul_rn(v[i], xs[l[i]]);
+    for (int i = 0; i < kItems / 2; i++) {
+      f64x2 pr;
+      pr.x = mul_rn(v[2 * i], xs[l[2 * i]]);
+      pr.y = mul_rn(v[2 * i + 1], xs[l[2 * i + 1]]);
+      *reinterpret_cast<f64x2 *>(prod + 512 * i + 2 * t) = pr;  // 16-byte LDS stores
+    }
     Cursor nxt = cur;
     nxt.k0 += kChunk;
     if (nxt.k0 >= cur.len) seek(nxt, cur.s + 1, s1, slice_off);

@@ -78,7 +78,7 @@ __device__ __forceinline__ void row_block(int w, const BatchItem &it, const Batc
       double tot = 0.0;
 #pragma unroll
       for (int q = 0; q < BLOCK / 64; q++) tot += wsum[q];
-      partials[2 * w] = tot;
+      partials[it.pslot] = tot;
     }
     __syncthreads();
     return;
@@ -106,12 +106,90 @@ __device__ __forceinline__ void row_block(int w, const BatchItem &it, const Batc
     const double s = ysh[i];
     const bool edge = (i == 0 || i == nrows - 1);
     if (edge && (rp[r] < lo || rp[r + 1] > hi)) {
-      partials[2 * w + (i == 0 ? 0 : 1)] = s;
+      partials[it.pslot + (i == 0 ? 0 : 1)] = s;
     } else {
       y[r] = b_axpby(M.alpha, s, M.beta, y[r]);
     }
   }
   __syncthreads();
+}
+
+// One row block with K right-hand sides (X n x K, Y m x K, row-major): lane
+// t works on vector t % K.  A block of one row (or a piece of one) spreads it
+// over all BLOCK/K lane groups and reduces across the workgroup in a fixed
+// order; otherwise each row gets max(subwave, K) lanes.  Cut rows leave K
+// partials per slot: partials[pslot + i*K + vec], i = 0 first row, 1 last.
+template <int K, int BLOCK>
+__device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchMatrix &M,
+                                               double *__restrict__ partials, double *lds) {
+  const int t = threadIdx.x;
+  const int vec = t % K;
+  const int *__restrict__ rp = M.rp;
+  const int *__restrict__ col = M.col;
+  const double *__restrict__ X = M.x;
+  double *__restrict__ Y = M.y;
+  const int lo = it.lo, hi = it.hi;
+  // sum over j = j0, j0+step, ... < hi, four elements' loads ahead of their gathers
+  auto dot = [&](int j0, int j1, int step) {
+    double sum = 0.0;
+    int j = j0;
+    for (; j + 3 * step < j1; j += 4 * step) {
+      double v[4];
+      int c[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        v[i] = b_val(M.val, M.val_type, j + i * step);
+        c[i] = __builtin_nontemporal_load(col + j + i * step);
+      }
+      double xv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) xv[i] = X[static_cast<int64_t>(c[i]) * K + vec];
+#pragma unroll
+      for (int i = 0; i < 4; i++) sum += v[i] * xv[i];
+    }
+    for (; j < j1; j += step)
+      sum += b_val(M.val, M.val_type, j) * X[static_cast<int64_t>(col[j]) * K + vec];
+    return sum;
+  };
+  const int nrows = it.row_end - it.row_begin;
+  if (nrows == 1) {
+    constexpr int G = BLOCK / K;
+    double sum = dot(lo + t / K, hi, G);
+#pragma unroll
+    for (int off = 32; off >= K; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if ((t & 63) < K) lds[(t >> 6) * K + vec] = sum;
+    __syncthreads();
+    if (t < K) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < BLOCK / 64; w++) s += lds[w * K + t];
+      const int r = it.row_begin;
+      if (rp[r] < lo || rp[r + 1] > hi) {
+        partials[it.pslot + t] = s;
+      } else {
+        double *yp = Y + static_cast<int64_t>(r) * K + t;
+        *yp = b_axpby(M.alpha, s, M.beta, *yp);
+      }
+    }
+    __syncthreads();
+    return;
+  }
+  const int sub = M.subwave < K ? K : M.subwave;  // power of two in [K, 64]
+  const int subs = BLOCK / sub;
+  const int grp = (t % sub) / K;
+  for (int r = it.row_begin + t / sub; r < it.row_end; r += subs) {
+    const int rs = rp[r], re = rp[r + 1];
+    double sum = dot(max(rs, lo) + grp, min(re, hi), sub / K);
+    for (int off = sub >> 1; off >= K; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if (grp == 0) {
+      if (rs < lo || re > hi) {
+        partials[it.pslot + (r == it.row_begin ? 0 : K) + vec] = sum;
+      } else {
+        double *yp = Y + static_cast<int64_t>(r) * K + vec;
+        *yp = b_axpby(M.alpha, sum, M.beta, *yp);
+      }
+    }
+  }
 }
 
 // One row panel of a matrix in column-panel form (16-bit slice offsets).
@@ -125,19 +203,19 @@ __device__ __forceinline__ void row_panel_any(int w, const BatchItem &it, const 
       panel::row_panel<double, RPT, uint16_t>(
           p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
           static_cast<const double *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
-          partials + 2 * w, lds);
+          partials + it.pslot, lds);
       break;
     case SBLAS_VAL_F32:
       panel::row_panel<float, RPT, uint16_t>(
           p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
           static_cast<const float *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
-          partials + 2 * w, lds);
+          partials + it.pslot, lds);
       break;
     default:
       panel::row_panel<uint16_t, RPT, uint16_t>(
           p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
           static_cast<const uint16_t *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
-          partials + 2 * w, lds);
+          partials + it.pslot, lds);
       break;
   }
 }
@@ -170,10 +248,18 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
       if (w >= end) break;  // counters only grow: every workgroup gets here
       const BatchItem it = items[w];
       const BatchMatrix &M = mats[it.mat];
-      if (it.kind == kBatchPanel)
+      if (it.kind == kBatchPanel) {
         row_panel_any(w, it, M, partials, lds);
-      else
+      } else if (M.k <= 1) {
         row_block<ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+      } else {
+        switch (M.k) {  // 2, 4, 8 or 16 (checked when the batch is created)
+          case 2: row_block_mrhs<2, BLOCK>(it, M, partials, lds); break;
+          case 4: row_block_mrhs<4, BLOCK>(it, M, partials, lds); break;
+          case 8: row_block_mrhs<8, BLOCK>(it, M, partials, lds); break;
+          default: row_block_mrhs<16, BLOCK>(it, M, partials, lds); break;
+        }
+      }
     }
   }
   if (t == 0) {
@@ -186,18 +272,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
   (void)nitems;
 }
 
-// Split rows of all matrices: one thread per row, partials in block order.
+// Split rows of all matrices: one thread per row (all its vectors), partials
+// in item order.
 __global__ void k_batch_merge(int groups, const int *__restrict__ gmat,
                               const int *__restrict__ grow, const int *__restrict__ gptr,
                               const int *__restrict__ gslot, const BatchMatrix *__restrict__ mats,
                               const double *__restrict__ partials) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= groups) return;
-  double s = 0.0;
-  for (int q = gptr[g]; q < gptr[g + 1]; q++) s += partials[gslot[q]];
   const BatchMatrix &M = mats[gmat[g]];
-  double *yp = M.y + grow[g];
-  *yp = b_axpby(M.alpha, s, M.beta, *yp);
+  const int k = M.k <= 1 ? 1 : M.k;
+  for (int v = 0; v < k; v++) {
+    double s = 0.0;
+    for (int q = gptr[g]; q < gptr[g + 1]; q++) s += partials[gslot[q] + v];
+    double *yp = M.y + static_cast<int64_t>(grow[g]) * k + v;
+    *yp = b_axpby(M.alpha, s, M.beta, *yp);
+  }
 }
 
 }  // namespace

@@ -8,9 +8,10 @@
 // columns.  A workgroup owns one row panel and walks its column panels in
 // increasing order: it stages x[q*W, (q+1)*W) in LDS with coalesced loads
 // (one pass over the panel's x lines), then streams that slice's elements in
-// chunks of 2048 (each thread eight consecutive elements, read with aligned
-// 16-byte loads: slices are padded to 8 elements), multiplies each by its x
-// from LDS and leaves the products in LDS.  The plan stores each
+// chunks of 2048 (each thread four pairs of consecutive elements, so every
+// load instruction of a wave reads whole lines; slices are padded to 8
+// elements to keep pairs aligned), multiplies each by its x from LDS and
+// leaves the products in LDS.  The plan stores each
 // (row panel, column panel) slice contiguously, rows in order, with 16-bit
 // column offsets inside the panel and row offsets inside the slice (16-bit
 // when every slice holds at most 65535 elements, else 32-bit).

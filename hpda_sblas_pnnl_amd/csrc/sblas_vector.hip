@@ -252,6 +252,34 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
 // leave partials (slot 2b for the block's first row, 2b+1 for its last) for
 // the ordered merge, so a row is never the work of a single sub-wave however
 // long it is.
+// sum over j = j0, j0+step, ... < j1 of val[j] * X[col[j]*K + vec]; four
+// elements' loads are issued before their gathers, so each lane keeps several
+// X rows in flight instead of one dependent chain per element
+template <typename VT, int K>
+__device__ __forceinline__ double strided_dot(int j0, int j1, int step,
+                                              const VT *__restrict__ val,
+                                              const int *__restrict__ col,
+                                              const double *__restrict__ X, int vec) {
+  double sum = 0.0;
+  int j = j0;
+  for (; j + 3 * step < j1; j += 4 * step) {
+    double v[4];
+    int c[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      v[i] = load_val(val, j + i * step);
+      c[i] = load_col(col, j + i * step);
+    }
+    double xv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) xv[i] = X[static_cast<int64_t>(c[i]) * K + vec];
+#pragma unroll
+    for (int i = 0; i < 4; i++) sum += v[i] * xv[i];
+  }
+  for (; j < j1; j += step) sum += load_val(val, j) * X[static_cast<int64_t>(load_col(col, j)) * K + vec];
+  return sum;
+}
+
 template <typename VT, int K, int SUBW, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_csr_blocks_mrhs(
     const RowBlock *__restrict__ blocks, int nblocks, int xcd_remap,
@@ -268,9 +296,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks_mrhs(
   if (rb.row_end - rb.row_begin == 1) {
     constexpr int G = BLOCK / K;
     const int g = t / K;
-    double sum = 0.0;
-    for (int j = rb.lo + g; j < rb.hi; j += G)
-      sum += load_val(val, j) * X[static_cast<int64_t>(load_col(col, j)) * K + vec];
+    double sum = strided_dot<VT, K>(rb.lo + g, rb.hi, G, val, col, X, vec);
 #pragma unroll
     for (int off = 32; off >= K; off >>= 1) sum += __shfl_xor(sum, off, 64);
     if ((t & 63) < K) red[t >> 6][vec] = sum;
@@ -296,9 +322,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks_mrhs(
   for (int r = rb.row_begin + sub; r < rb.row_end; r += SUBS) {
     const int rs = rp[r], re = rp[r + 1];
     const int s = max(rs, rb.lo), e = min(re, rb.hi);
-    double sum = 0.0;
-    for (int j = s + grp; j < e; j += STEP)
-      sum += load_val(val, j) * X[static_cast<int64_t>(load_col(col, j)) * K + vec];
+    double sum = strided_dot<VT, K>(s + grp, e, STEP, val, col, X, vec);
 #pragma unroll
     for (int off = SUBW / 2; off >= K; off >>= 1) sum += __shfl_xor(sum, off, SUBW);
     if (grp == 0) {

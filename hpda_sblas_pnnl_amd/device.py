@@ -62,7 +62,8 @@ def _stream_ptr(stream):
 class SpmvBatch:
     """Many CSR matrices run as one persistent launch (optionally a replayed
     hipGraph).  items: list of dicts with keys m, n, row_ptr, col, val, x, y,
-    alpha, beta (tensors on one HIP device).  variant: 0 lets the autotuner
+    alpha, beta (tensors on one HIP device) and optionally k: x (n x k) and y
+    (m x k) row-major hold k right-hand sides (k in 1, 2, 4, 8, 16).  variant: 0 lets the autotuner
     pick row blocks or column panels per matrix, 2 forces row blocks, 3 asks
     for column panels wherever the format allows them."""
 
@@ -77,6 +78,7 @@ class SpmvBatch:
             arr[i].d_col = it["col"].data_ptr()
             arr[i].d_val = it["val"].data_ptr()
             arr[i].val_type = val_type_of(it["val"])
+            arr[i].k = int(it.get("k", 1))
             arr[i].d_x = it["x"].data_ptr()
             arr[i].d_y = it["y"].data_ptr()
             arr[i].alpha = float(it.get("alpha", 1.0))

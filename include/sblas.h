@@ -261,7 +261,8 @@ typedef struct sblas_batch_item {
   const int *d_col;
   const void *d_val;
   int val_type;
-  int pad_;
+  int k;           /* right-hand sides: d_x n x k, d_y m x k, row-major;      */
+                   /* 0 or 1 = one vector, else 2, 4, 8 or 16                  */
   const double *d_x;
   double *d_y;
   double alpha, beta;
