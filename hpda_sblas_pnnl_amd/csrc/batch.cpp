@@ -192,7 +192,9 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     if (kv == 1 && it.m > 0 && it.nnz > 0 && want != 1 && want != 2 &&
         (want == 3 || rs.max_row <= 8192)) {
       const std::vector<ExecChunk> one{ExecChunk{0, it.nnz, 0, it.m}};
-      const int rows = (opts && opts->panel_rows > 0) ? opts->panel_rows : 1024;
+      const int rows = (opts && opts->panel_rows > 0)
+                           ? std::min(opts->panel_rows, kBatchPanelRowsMax)
+                           : kBatchPanelRowsMax;
       int st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col,
                            it.d_val, one, h_rp.data(), want == 3, 1, f, &panel, nullptr);
       if (st == SBLAS_OK && panel && !f.srp) panel = false;  // slices beyond 16-bit offsets

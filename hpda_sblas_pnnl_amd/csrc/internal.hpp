@@ -204,6 +204,7 @@ struct BatchItem {  // one row block, or one row panel, of one matrix
 // the last one resets them all).  Each item's arithmetic does not depend on
 // which workgroup runs it.
 constexpr int kBatchQueues = 8;  // one per XCD
+constexpr int kBatchPanelRowsMax = 1024;  // rows per row panel in a batch (4 per thread)
 int launch_batch(int items_per_thread, int grid, int nitems, const BatchItem *items,
                  const BatchMatrix *mats, double *partials, const int *qbound, int *ctr,
                  hipStream_t s);

@@ -195,7 +195,7 @@ __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchM
 // One row panel of a matrix in column-panel form (16-bit slice offsets).
 __device__ __forceinline__ void row_panel_any(int w, const BatchItem &it, const BatchMatrix &M,
                                               double *__restrict__ partials, double *lds) {
-  constexpr int RPT = kPanelRowsMax / 256;
+  constexpr int RPT = kBatchPanelRowsMax / 256;
   const int p = it.row_begin;
   const PanelInfo pi = M.pinfo[p];
   switch (M.val_type) {
@@ -240,12 +240,15 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
   for (int d = 0; d < kBatchQueues; d++) {
     const int q = (home + d) & (kBatchQueues - 1);
     const int end = qbound[q + 1];
+    if (t == 0) next = qbound[q] + atomicAdd(&ctr[q], 1);
+    __syncthreads();
+    int w = next;
     for (;;) {
-      if (t == 0) next = qbound[q] + atomicAdd(&ctr[q], 1);
-      __syncthreads();
-      const int w = next;
       __syncthreads();  // every thread has read `next` before it is taken again
       if (w >= end) break;  // counters only grow: every workgroup gets here
+      // claim the following item now; its counter round trip overlaps this item
+      int claimed = 0;
+      if (t == 0) claimed = qbound[q] + atomicAdd(&ctr[q], 1);
       const BatchItem it = items[w];
       const BatchMatrix &M = mats[it.mat];
       if (it.kind == kBatchPanel) {
@@ -260,6 +263,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
           default: row_block_mrhs<16, BLOCK>(it, M, partials, lds); break;
         }
       }
+      if (t == 0) next = claimed;  // last read before this iteration's first barrier
+      __syncthreads();
+      w = next;
     }
   }
   if (t == 0) {
