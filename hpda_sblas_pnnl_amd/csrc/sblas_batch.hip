@@ -18,11 +18,14 @@
 namespace sblas {
 namespace {
 
-__device__ __forceinline__ double b_val(const void *p, int type, int j) {
-  if (type == SBLAS_VAL_F64) return __builtin_nontemporal_load(static_cast<const double *>(p) + j);
-  if (type == SBLAS_VAL_F32)
-    return static_cast<double>(__builtin_nontemporal_load(static_cast<const float *>(p) + j));
-  const uint32_t h = __builtin_nontemporal_load(static_cast<const uint16_t *>(p) + j);
+__device__ __forceinline__ double t_val(const double *p, int j) {
+  return __builtin_nontemporal_load(p + j);
+}
+__device__ __forceinline__ double t_val(const float *p, int j) {
+  return static_cast<double>(__builtin_nontemporal_load(p + j));
+}
+__device__ __forceinline__ double t_val(const uint16_t *p, int j) {
+  const uint32_t h = __builtin_nontemporal_load(p + j);
   return static_cast<double>(__uint_as_float(h << 16));
 }
 
@@ -33,10 +36,11 @@ __device__ __forceinline__ double b_axpby(double alpha, double s, double beta, d
 // One row block (the LDS-staged form of kernel 1): products of the block's
 // element window into LDS, sub-wave row sums, y written (or the partials of
 // rows the window cuts).  lds holds BLOCK*ITEMS + kStreamRowsMax doubles.
-template <int ITEMS, int BLOCK>
+template <typename VT, int ITEMS, int BLOCK>
 __device__ __forceinline__ void row_block(int w, const BatchItem &it, const BatchMatrix &M,
                                           double *__restrict__ partials, double *lds,
                                           double *wsum) {
+  const VT *__restrict__ val = static_cast<const VT *>(M.val);
   constexpr int CAP = BLOCK * ITEMS;
   double *prod = lds;
   double *ysh = lds + CAP;
@@ -53,7 +57,7 @@ __device__ __forceinline__ void row_block(int w, const BatchItem &it, const Batc
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
       const int e = min(lo + t + k * BLOCK, hi - 1);
-      v[k] = b_val(M.val, M.val_type, e);
+      v[k] = t_val(val, e);
       c[k] = __builtin_nontemporal_load(col + e);
     }
     double xv[ITEMS];
@@ -119,9 +123,10 @@ __device__ __forceinline__ void row_block(int w, const BatchItem &it, const Batc
 // over all BLOCK/K lane groups and reduces across the workgroup in a fixed
 // order; otherwise each row gets max(subwave, K) lanes.  Cut rows leave K
 // partials per slot: partials[pslot + i*K + vec], i = 0 first row, 1 last.
-template <int K, int BLOCK>
+template <typename VT, int K, int BLOCK>
 __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchMatrix &M,
                                                double *__restrict__ partials, double *lds) {
+  const VT *__restrict__ val = static_cast<const VT *>(M.val);
   const int t = threadIdx.x;
   const int vec = t % K;
   const int *__restrict__ rp = M.rp;
@@ -138,7 +143,7 @@ __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchM
       int c[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        v[i] = b_val(M.val, M.val_type, j + i * step);
+        v[i] = t_val(val, j + i * step);
         c[i] = __builtin_nontemporal_load(col + j + i * step);
       }
       double xv[4];
@@ -148,7 +153,7 @@ __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchM
       for (int i = 0; i < 4; i++) sum += v[i] * xv[i];
     }
     for (; j < j1; j += step)
-      sum += b_val(M.val, M.val_type, j) * X[static_cast<int64_t>(col[j]) * K + vec];
+      sum += t_val(val, j) * X[static_cast<int64_t>(col[j]) * K + vec];
     return sum;
   };
   const int nrows = it.row_end - it.row_begin;
@@ -193,30 +198,42 @@ __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchM
 }
 
 // One row panel of a matrix in column-panel form (16-bit slice offsets).
-__device__ __forceinline__ void row_panel_any(int w, const BatchItem &it, const BatchMatrix &M,
+__device__ __forceinline__ void row_panel_any(const BatchItem &it, const BatchMatrix &M,
                                               double *__restrict__ partials, double *lds) {
   constexpr int RPT = kBatchPanelRowsMax / 256;
   const int p = it.row_begin;
   const PanelInfo pi = M.pinfo[p];
+  double *slots = partials + it.pslot;
   switch (M.val_type) {
     case SBLAS_VAL_F64:
-      panel::row_panel<double, RPT, uint16_t>(
-          p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
-          static_cast<const double *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
-          partials + it.pslot, lds);
+      panel::row_panel<double, RPT, uint16_t>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
+                                              M.srp, static_cast<const double *>(M.pval), M.pcol,
+                                              M.x, M.y, M.alpha, M.beta, slots, lds);
       break;
     case SBLAS_VAL_F32:
-      panel::row_panel<float, RPT, uint16_t>(
-          p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
-          static_cast<const float *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
-          partials + it.pslot, lds);
+      panel::row_panel<float, RPT, uint16_t>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
+                                             M.srp, static_cast<const float *>(M.pval), M.pcol,
+                                             M.x, M.y, M.alpha, M.beta, slots, lds);
       break;
     default:
-      panel::row_panel<uint16_t, RPT, uint16_t>(
-          p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
-          static_cast<const uint16_t *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta,
-          partials + it.pslot, lds);
+      panel::row_panel<uint16_t, RPT, uint16_t>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
+                                                M.srp, static_cast<const uint16_t *>(M.pval),
+                                                M.pcol, M.x, M.y, M.alpha, M.beta, slots, lds);
       break;
+  }
+}
+
+// A row-block item of one value type: one vector or K right-hand sides.
+template <typename VT, int ITEMS, int BLOCK>
+__device__ __forceinline__ void rows_typed(int w, const BatchItem &it, const BatchMatrix &M,
+                                           double *__restrict__ partials, double *lds,
+                                           double *wsum) {
+  switch (M.k) {  // 1, 2, 4, 8 or 16 (checked when the batch is created)
+    case 2: row_block_mrhs<VT, 2, BLOCK>(it, M, partials, lds); break;
+    case 4: row_block_mrhs<VT, 4, BLOCK>(it, M, partials, lds); break;
+    case 8: row_block_mrhs<VT, 8, BLOCK>(it, M, partials, lds); break;
+    case 16: row_block_mrhs<VT, 16, BLOCK>(it, M, partials, lds); break;
+    default: row_block<VT, ITEMS, BLOCK>(w, it, M, partials, lds, wsum); break;
   }
 }
 
@@ -252,16 +269,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
       const BatchItem it = items[w];
       const BatchMatrix &M = mats[it.mat];
       if (it.kind == kBatchPanel) {
-        row_panel_any(w, it, M, partials, lds);
-      } else if (M.k <= 1) {
-        row_block<ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+        row_panel_any(it, M, partials, lds);
+      } else if (M.val_type == SBLAS_VAL_F64) {  // value type chosen once per item
+        rows_typed<double, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+      } else if (M.val_type == SBLAS_VAL_F32) {
+        rows_typed<float, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
       } else {
-        switch (M.k) {  // 2, 4, 8 or 16 (checked when the batch is created)
-          case 2: row_block_mrhs<2, BLOCK>(it, M, partials, lds); break;
-          case 4: row_block_mrhs<4, BLOCK>(it, M, partials, lds); break;
-          case 8: row_block_mrhs<8, BLOCK>(it, M, partials, lds); break;
-          default: row_block_mrhs<16, BLOCK>(it, M, partials, lds); break;
-        }
+        rows_typed<uint16_t, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
       }
       if (t == 0) next = claimed;  // last read before this iteration's first barrier
       __syncthreads();
