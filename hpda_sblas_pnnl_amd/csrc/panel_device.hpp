@@ -159,18 +159,20 @@ __device__ __forceinline__ void row_panel(
     __syncthreads();  // products visible; x of the current slice no longer read
     const bool new_slice = nxt.s != cur.s && nxt.s < s1;
     if (new_slice) stage_x(nxt.s);
-    // each row adds its products of this chunk in element order; eight LDS
-    // reads (clamped inside the row) are in flight before the ordered adds
+    // each row adds its products of this chunk in element order, the next
+    // product's LDS read issued ahead of each add
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       const int lo = max(b[j], cur.k0) - cur.k0, hi = min(e[j], cur.k0 + kChunk) - cur.k0;
       double a = acc[j];
-      for (int k = lo; k < hi; k += 8) {
-        double q[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) q[i] = prod[min(k + i, hi - 1)];
-#pragma unroll
-        for (int i = 0; i < 8; i++) a = (k + i < hi) ? add_rn(a, q[i]) : a;
+      if (lo < hi) {
+        double q = prod[lo];
+        for (int k = lo + 1; k < hi; k++) {
+          const double nq = prod[k];
+          a = add_rn(a, q);
+          q = nq;
+        }
+        a = add_rn(a, q);
       }
       acc[j] = a;
     }
