@@ -333,8 +333,8 @@ def load_traffic(workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="banded", choices=["banded", "rmat", "powerlaw"])
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--chunks", type=int, default=0)

@@ -99,14 +99,18 @@ __device__ __forceinline__ void row_panel(
 #pragma unroll
   for (int j = 0; j < RPT; j++) acc[j] = 0.0;
 
-  // x[q*W, (q+1)*W) of slice s into LDS
+  // x[q*W, (q+1)*W) of slice s into LDS, two columns per thread and step
+  // (one 16-byte LDS store); columns past n read as 0
   auto stage_x = [&](int s) {
     const int c0 = (qb + (s - s0)) * W;
 #pragma unroll
-    for (int i = 0; i < W / 256; i++) {
-      const int c = c0 + t + i * 256;
-      const double xv = x[min(c, n - 1)];  // no load under a branch
-      xs[t + i * 256] = (c < n) ? xv : 0.0;
+    for (int i = 0; i < W / 512; i++) {
+      const int c = c0 + 2 * t + i * 512;
+      const double x0 = x[min(c, n - 1)], x1 = x[min(c + 1, n - 1)];  // no load under a branch
+      f64x2 q;
+      q.x = (c < n) ? x0 : 0.0;
+      q.y = (c + 1 < n) ? x1 : 0.0;
+      *reinterpret_cast<f64x2 *>(xs + 2 * t + i * 512) = q;
     }
   };
   // element ranges [b, e) of this thread's rows inside slice s
