@@ -196,7 +196,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
                            ? std::min(opts->panel_rows, kBatchPanelRowsMax)
                            : kBatchPanelRowsMax;
       int st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col,
-                           it.d_val, one, h_rp.data(), want == 3, 1, false, f, &panel, nullptr);
+                           it.d_val, one, h_rp.data(), want == 3, 1, f, &panel, nullptr);
       if (st == SBLAS_OK && panel && !f.srp) panel = false;  // slices beyond 16-bit offsets
       if (st == SBLAS_ERR_UNSUPPORTED && want != 3) {
         clear_error();

@@ -126,10 +126,6 @@ int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha
 constexpr int kPanelCols = 4096;  // x panel staged in LDS (32 KB of FP64)
 constexpr int kPanelRowsMax = 2048;  // rows per row panel (8 per thread)
 constexpr int kPanelPad = 8;         // slices start on 8-element boundaries
-constexpr int kSegThreads = 256;     // segmented form: threads per workgroup
-constexpr int kSegItems = 16;        // segmented form: elements per thread per chunk
-constexpr int kSegChunk = kSegThreads * kSegItems;  // elements per transposed chunk
-constexpr int kSegRowsMax = 1536;    // segmented form: rows per row panel
 // One row panel: rows [row0, row0 + nrow) of one execution chunk, its first
 // column panel, and flags: bit 0 = the first row is cut by the chunk's
 // element window (partial slot 2p), bit 1 = the last row is (slot 2p+1).
@@ -151,21 +147,15 @@ struct PanelFormat {
   uint16_t *srp = nullptr;
   uint32_t *srp32 = nullptr;
   void *pval = nullptr;         // values, slice-major, slices padded to kPanelPad
-                                // (kSegItems in the segmented form)
   uint16_t *pcol = nullptr;     // column offsets inside the column panel (padding: 0)
   double *partials = nullptr;   // 2 slots per row panel
   int64_t padded = 0;           // elements stored, slices padded
-  // segmented form (order-relaxed sums): chunks stored transposed, one flag
-  // bit per element that starts a row, the rows with elements of each slice
-  bool seg = false;
-  uint8_t *flags = nullptr;     // bit (e & 7) of byte e >> 3: element e starts a row
-  uint16_t *rownz = nullptr;    // per slice (stride rows): rows with elements, in order
   std::vector<ExecChunk> hpanels;  // host copy: element window and rows of each panel
   std::vector<void *> buffers;
 };
 int panel_rpt(int rows);
 // Rows per row panel so the panels fill whole rounds of resident workgroups.
-int panel_rows_for(int val_type, int m, int device, bool seg);
+int panel_rows_for(int val_type, int m, int device);
 // Builds the format on the device: each execution chunk of `ex` cut into row
 // panels of at most `rows` rows (h_rp: host copy of the row pointers).  With
 // force == false it stops after the per-panel column ranges and sets
@@ -173,8 +163,7 @@ int panel_rows_for(int val_type, int m, int device, bool seg);
 // slices, or rows whose x window is narrow).
 int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp, const int *col,
                 const void *val, const std::vector<ExecChunk> &ex, const int *h_rp,
-                bool force, int min_panels, bool seg, PanelFormat &f, bool *chosen,
-                hipStream_t s);
+                bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s);
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
                       const double *x, double *y, double alpha, double beta, hipStream_t s);
 

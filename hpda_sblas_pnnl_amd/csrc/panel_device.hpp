@@ -4,8 +4,6 @@
 // sblas_panel.hip.
 #pragma once
 
-#include <type_traits>
-
 #include "internal.hpp"
 
 namespace sblas {
@@ -201,273 +199,6 @@ __device__ __forceinline__ void row_panel(
       }
     }
   }
-}
-
-// ---- segmented form (order-relaxed) ----------------------------------------
-
-// Each chunk of kSegChunk = 256*E elements of a slice (E = kSegItems; elements
-// in row-major order, the last chunk shorter, padded to E) is stored
-// transposed in pairs: thread t of the chunk owns elements E*t .. E*t+E-1, its
-// pair i (elements E*t+2i, E*t+2i+1) stored at chunk_base + 2*(i*T + t)
-// (T = the chunk's length / E), so every load instruction reads T consecutive
-// pairs.  One flag word of E bits per
-// thread marks which of its elements start a row.  A thread
-// sums its products in registers; the value of a row that crosses threads is
-// combined by a segmented scan over the wave on DPP lane moves, across the
-// four waves through summaries in LDS, and across chunks in registers, so each
-// row of a slice is closed exactly once and added to its running sum in LDS.
-// The order of additions inside a row is fixed (results repeat bit for bit
-// from run to run) but differs from the host loop's sequential order.
-
-static_assert(kSegItems == 8 || kSegItems == 16, "a thread's flags are one byte or one 16-bit word");
-static_assert(kPanelPad % 2 == 0 && kSegItems % kPanelPad == 0, "segmented slices hold whole groups");
-struct SegWave {                   // one wave's part of a chunk
-  double open;                     // value of the row open at the wave's end (if has)
-  double tot;                      // sum carried through a wave without row starts
-  int cnt;                         // row starts in the wave
-  int has;
-};
-constexpr int kSegWaves = kSegThreads / 64;
-constexpr int kSegLdsBytes = (kPanelCols + kSegRowsMax) * 8 + 2 * kSegWaves * sizeof(SegWave) +
-                             2 * kSegRowsMax * 2;
-static_assert(kPanelCols % (2 * kSegThreads) == 0, "x panel staged in whole pairs");
-
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_d(double v) {
-  return __builtin_amdgcn_update_dpp(0.0, v, CTRL, ROWS, 0xf, false);
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ int dpp_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
-}
-// inclusive prefix sum over the 64 lanes (row_shr inside 16-lane rows, then
-// row_bcast15 / row_bcast31 across them; missing sources read 0)
-__device__ __forceinline__ int seg_wave_incl(int v) {
-  v += dpp_i<0x111, 0xf>(v);
-  v += dpp_i<0x112, 0xf>(v);
-  v += dpp_i<0x114, 0xf>(v);
-  v += dpp_i<0x118, 0xf>(v);
-  v += dpp_i<0x142, 0xa>(v);
-  v += dpp_i<0x143, 0xc>(v);
-  return v;
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step(double &v, int &head) {
-  const double vu = dpp_d<CTRL, ROWS>(v);
-  const int hu = dpp_i<CTRL, ROWS>(head);
-  if (!head) {
-    v += vu;
-    head = hu;
-  }
-}
-// inclusive segmented sum over the 64 lanes; head = 1 opens a segment
-__device__ __forceinline__ double seg_wave_sum(double v, int head) {
-  seg_step<0x111, 0xf>(v, head);
-  seg_step<0x112, 0xf>(v, head);
-  seg_step<0x114, 0xf>(v, head);
-  seg_step<0x118, 0xf>(v, head);
-  seg_step<0x142, 0xa>(v, head);
-  seg_step<0x143, 0xc>(v, head);
-  return v;
-}
-
-// Adds a closed row's value of one slice to the row's sum in LDS.  A row is
-// closed once per slice and slices are separated by barriers, so the adds to
-// one row come in slice order; the LDS atomic (no value returned) keeps the
-// read of the old sum off the thread's dependency chain.
-__device__ __forceinline__ void seg_close(double *s, double v) {
-  __hip_atomic_fetch_add(s, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Row panel p in the segmented form by one workgroup of kSegThreads threads.  lds
-// holds kSegLdsBytes; R <= kSegRowsMax.  Same slot / y conventions as
-// row_panel.  The workgroup leaves through a barrier.
-template <typename VT>
-__device__ __forceinline__ void row_panel_seg(
-    int p, int n, int R, const PanelInfo &pi, const int *__restrict__ slice_base,
-    const int64_t *__restrict__ slice_off, const uint8_t *__restrict__ flags,
-    const uint16_t *__restrict__ rownz, const VT *__restrict__ pval,
-    const uint16_t *__restrict__ pcol, const double *__restrict__ x, double *__restrict__ y,
-    double alpha, double beta, double *__restrict__ slots, char *lds) {
-  constexpr int W = kPanelCols;
-  double *xs = reinterpret_cast<double *>(lds);
-  double *sum = xs + W;
-  SegWave *ws = reinterpret_cast<SegWave *>(sum + kSegRowsMax);  // two sets, one per wave
-  uint16_t *rzb = reinterpret_cast<uint16_t *>(ws + 2 * kSegWaves);  // two slices' row ids
-  constexpr int NT = kSegThreads;
-  const int t = threadIdx.x;
-  const int wave = t >> 6, lane = t & 63;
-  const int s0 = slice_base[p], s1 = slice_base[p + 1];
-  const int qb = pi.q_lo;
-  for (int r = t; r < R; r += NT) sum[r] = 0.0;
-
-  constexpr int E = kSegItems;
-  typedef typename std::conditional<E == 16, uint16_t, uint8_t>::type FlagT;
-  const FlagT *__restrict__ fw = reinterpret_cast<const FlagT *>(flags);
-  double v[E];
-  int c[E], f = 0;
-  auto stage = [&](int s, uint16_t *rz) {  // x and the slice's rows with elements
-    const int c0 = (qb + (s - s0)) * W;
-#pragma unroll
-    for (int i = 0; i < W / (2 * NT); i++) {
-      const int cc = c0 + 2 * t + i * 2 * NT;
-      const double x0 = x[min(cc, n - 1)], x1 = x[min(cc + 1, n - 1)];  // no load under a branch
-      f64x2 q;
-      q.x = (cc < n) ? x0 : 0.0;
-      q.y = (cc + 1 < n) ? x1 : 0.0;
-      *reinterpret_cast<f64x2 *>(xs + 2 * t + i * 2 * NT) = q;
-    }
-    for (int i = t; i < R; i += NT) rz[i] = rownz[static_cast<int64_t>(s) * R + i];
-  };
-  auto load_chunk = [&](const Cursor &cr) {
-    const int T = min(kSegChunk, cr.len - cr.k0) / E;
-    const int tt = min(t, T - 1);  // threads past the chunk re-read its last group
-    const int64_t cb = cr.base + cr.k0;
-    const VT *__restrict__ vb = pval + cb;
-    const uint16_t *__restrict__ cbp = pcol + cb;
-#pragma unroll
-    for (int i = 0; i < E / 2; i++) {  // pair i of the thread: one load each
-      const int k = 2 * (i * T + tt);
-      load2(vb + k, v[2 * i], v[2 * i + 1]);
-      load2_cols(cbp + k, c[2 * i], c[2 * i + 1]);
-    }
-    f = fw[cb / E + tt];
-    if (t >= T) f = 0;
-  };
-  auto advance = [&](Cursor cr) {
-    cr.k0 += kSegChunk;
-    if (cr.k0 >= cr.len) seek(cr, cr.s + 1, s1, slice_off);
-    return cr;
-  };
-
-  Cursor cur;
-  seek(cur, s0, s1, slice_off);
-  int rb = 0;  // row-id buffer of the current slice
-  if (cur.s < s1) {
-    stage(cur.s, rzb);
-    load_chunk(cur);
-  }
-  __syncthreads();
-  double carry = 0.0;  // value of the row open when the chunk begins
-  int rank0 = 0;       // row starts in the slice before the chunk
-  int buf = 0;
-  while (cur.s < s1) {
-    const int T = min(kSegChunk, cur.len - cur.k0) / E;
-    const bool act = t < T;
-    double pr[E];
-#pragma unroll
-    for (int i = 0; i < E; i++) {  // every gather issued, then the products selected
-      const double xv = xs[c[i] & (W - 1)];  // padding (column 0xffff) reads an entry in range
-      const double pm = mul_rn(v[i], xv);
-      pr[i] = (act && c[i] != 0xffff) ? pm : 0.0;
-    }
-    const int fl = f;
-    // in-thread walk: the part before the first start (a) and after the last
-    double cur_s = 0.0, first = 0.0;
-    int seen = 0;
-#pragma unroll
-    for (int i = 0; i < E; i++) {
-      if ((fl >> i) & 1) {
-        if (seen == 0) first = cur_s;
-        seen++;
-        cur_s = 0.0;
-      }
-      cur_s += pr[i];
-    }
-    const bool hs = seen > 0;
-    const double a = hs ? first : cur_s;
-    const double last = hs ? cur_s : 0.0;
-    // wave scans: row-start counts, and the open row's value across lanes
-    const int incl = seg_wave_incl(seen);
-    const unsigned long long smask = __ballot(hs);
-    const int head = (lane == 0) ? 1 : static_cast<int>((smask >> (lane - 1)) & 1ull);
-    const double sv = seg_wave_sum(a, head);
-    const unsigned long long below = smask & ((1ull << lane) - 1ull);
-    const int prev = below ? 63 - __clzll(static_cast<long long>(below)) : 0;
-    const double prev_last = __shfl(last, prev, 64);
-    const int llast = smask ? 63 - __clzll(static_cast<long long>(smask)) : 0;
-    const double open_last = __shfl(last, llast, 64);
-    const double sv63 = __shfl(sv, 63, 64);
-    const int cnt63 = __shfl(incl, 63, 64);
-    if (lane == 0) {
-      SegWave w;
-      w.cnt = cnt63;
-      w.has = smask != 0ull;
-      w.open = open_last + (llast < 63 ? sv63 : 0.0);
-      w.tot = sv63;
-      ws[buf * kSegWaves + wave] = w;
-    }
-    const Cursor nxt = advance(cur);
-    if (nxt.s < s1) load_chunk(nxt);  // in flight during the closes below
-    const bool slice_end = nxt.s != cur.s;
-    __syncthreads();  // wave summaries visible; x of this chunk read
-    const uint16_t *rz = rzb + rb * kSegRowsMax;
-    // the next slice's x and row ids (row ids into the other buffer: this
-    // slice's are still read by the closes)
-    if (slice_end && nxt.s < s1) stage(nxt.s, rzb + (rb ^ 1) * kSegRowsMax);
-    double cin = carry, my_cin = 0.0;
-    int rk = rank0, my_rank = 0;
-#pragma unroll
-    for (int w2 = 0; w2 < kSegWaves; w2++) {
-      const SegWave w = ws[buf * kSegWaves + w2];
-      if (w2 == wave) {
-        my_cin = cin;
-        my_rank = rk;
-      }
-      cin = w.has ? w.open : cin + w.tot;
-      rk += w.cnt;
-    }
-    // the row closed by this thread's first start, then rows inside the thread
-    if (hs) {
-      const int rank = my_rank + incl - seen - 1;  // rank of the row open before the start
-      const double val = sv + (below ? prev_last : my_cin);
-      // a start at element i closes the row of rank (rank + starts before i);
-      // every row id read first (rank + j < the slice's row count <= R), then
-      // the adds
-      int rid[E];
-      int j = 0;
-#pragma unroll
-      for (int i = 0; i < E; i++) {
-        rid[i] = rz[max(rank + j, 0)];
-        j += (fl >> i) & 1;
-      }
-      double cs = 0.0;
-      j = 0;
-#pragma unroll
-      for (int i = 0; i < E; i++) {
-        if ((fl >> i) & 1) {
-          if (j > 0 || rank >= 0) seg_close(sum + rid[i], j > 0 ? cs : val);
-          j++;
-          cs = 0.0;
-        }
-        cs += pr[i];
-      }
-    }
-    carry = cin;
-    rank0 = rk;
-    buf ^= 1;
-    if (slice_end) {
-      if (t == 0 && rank0 > 0) seg_close(sum + rz[rank0 - 1], carry);  // the slice's last row
-      carry = 0.0;
-      rank0 = 0;
-      rb ^= 1;
-      __syncthreads();  // x and row ids of the next slice visible
-    }
-    cur = nxt;
-  }
-  __syncthreads();
-  for (int rl = t; rl < pi.nrow; rl += NT) {
-    const double s_ = sum[rl];
-    if (rl == 0 && (pi.flags & 1)) {
-      slots[0] = s_;
-    } else if (rl == pi.nrow - 1 && (pi.flags & 2)) {
-      slots[1] = s_;
-    } else {
-      const int r = pi.row0 + rl;
-      y[r] = (beta == 0.0) ? alpha * s_ : alpha * s_ + beta * y[r];
-    }
-  }
-  __syncthreads();
 }
 
 }  // namespace panel

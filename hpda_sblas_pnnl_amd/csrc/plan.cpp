@@ -413,7 +413,7 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     set_error(SBLAS_ERR_UNSUPPORTED, "multi-vector products need a CSR or CSC plan");
     return SBLAS_ERR_UNSUPPORTED;
   }
-  if (variant >= 3 && k == 1 && streams.empty()) {
+  if (variant == 3 && k == 1 && streams.empty()) {
     hipEvent_t *te = timing_slot();
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
     st = launch_panel_spmv(val_type, panel, m, n, vl.xcd_remap, X, Y, alpha, beta, s);
@@ -586,20 +586,17 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
   if (kernel == SBLAS_KERNEL_MERGE) return setup_tiles();
   if (kernel == SBLAS_KERNEL_CSR5) return setup_csr5(h_rp);
   // the panel form sums each row in one thread: not for matrices with very long rows
-  const bool panel_ok = opts.variant >= 3 || stats.max_row <= 8192;
-  if (variant == 2 && (opts.variant == 0 || opts.variant == 3 || opts.variant == 4) &&
-      opts.streams <= 1 && nnz > 0 && panel_ok) {
+  const bool panel_ok = opts.variant == 3 || stats.max_row <= 8192;
+  if (variant == 2 && (opts.variant == 0 || opts.variant == 3) && opts.streams <= 1 && nnz > 0 &&
+      panel_ok) {
     // column-panel form: forced by variant 3, else taken when it pays
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     const int min_panels = 2 * std::max(cus, 1);
-    // variant 4: the segmented (order-relaxed) form of the column panels
-    const bool seg = opts.variant == 4;
-    const int rows =
-        opts.panel_rows > 0 ? opts.panel_rows : panel_rows_for(val_type, m, device, seg);
+    const int rows = opts.panel_rows > 0 ? opts.panel_rows : panel_rows_for(val_type, m, device);
     bool chosen = false;
     st = panel_build(val_type, rows, m, n, nnz, a0, a1, val, exec, h_rp.data(),
-                     opts.variant >= 3, min_panels, seg, panel, &chosen, nullptr);
+                     opts.variant == 3, min_panels, panel, &chosen, nullptr);
     if (st == SBLAS_ERR_UNSUPPORTED && opts.variant == 0) {
       // a slice too large for 16-bit offsets: keep the row-block form
       for (void *p : panel.buffers) (void)hipFree(p);
@@ -611,7 +608,7 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
       return st;
     }
     if (chosen) {
-      variant = panel.seg ? 4 : 3;
+      variant = 3;
       int remap = opts.xcd_remap;
       if (remap < 0) remap = env_int("SBLAS_XCD_REMAP", 1);
       vl.xcd_remap = remap ? 1 : 0;
@@ -636,7 +633,7 @@ bool Plan::uses_tiles(int k) const {
 
 bool Plan::has_merge(int k) const {
   if (uses_tiles(k)) return tiles > 0;
-  if (variant >= 3 && k == 1 && streams.empty()) return pgroups > 0;
+  if (variant == 3 && k == 1 && streams.empty()) return pgroups > 0;
   if (variant >= 2 && streams.empty()) return bgroups > 0;  // row blocks (any k)
   return ngroups > 0;
 }
@@ -644,7 +641,7 @@ bool Plan::has_merge(int k) const {
 int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) {
   if (uses_tiles(k))
     return tiles > 0 ? launch_tile_fixup(carry_count(), d_carry, Y, alpha, beta, s) : SBLAS_OK;
-  if (variant >= 3 && k == 1 && streams.empty())
+  if (variant == 3 && k == 1 && streams.empty())
     return pgroups > 0 ? launch_merge(d_pmrows, d_pmptr, d_pmslots, pgroups, panel.partials, 1, Y,
                                       alpha, beta, s)
                        : SBLAS_OK;
@@ -927,8 +924,8 @@ int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info) {
   info->variant = p.variant;
   info->row_blocks = static_cast<int>(p.rblocks.size());
   info->items = p.sl.items;
-  info->panel_rows = p.variant >= 3 ? p.panel.rows : 0;
-  info->panel_slices = p.variant >= 3 ? p.panel.nslices : 0;
+  info->panel_rows = p.variant == 3 ? p.panel.rows : 0;
+  info->panel_slices = p.variant == 3 ? p.panel.nslices : 0;
   if (p.variant == 2) info->subwave = p.sl.subwave;
   return SBLAS_OK;
 }

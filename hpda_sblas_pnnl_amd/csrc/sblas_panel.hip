@@ -34,9 +34,9 @@
 namespace sblas {
 namespace {
 
-__global__ void k_fill_u32(int64_t count, uint32_t *__restrict__ a, uint32_t value = 0u) {
+__global__ void k_fill_u32(int64_t count, uint32_t *__restrict__ a) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < count) a[i] = value;
+  if (i < count) a[i] = 0u;
 }
 
 }  // namespace
@@ -60,26 +60,6 @@ __global__ __launch_bounds__(256) void k_panel_spmv(
   const PanelInfo pi = info[p];
   panel::row_panel<VT, RPT, OT>(p, n, R, pi, slice_base, slice_off, srp, pval, pcol, x, y, alpha,
                                 beta, partials + 2 * p, lds);
-}
-
-template <typename VT>
-__global__ __launch_bounds__(kSegThreads) void k_panel_spmv_seg(
-    int n, int R, int npanels, int xcd_remap, const PanelInfo *__restrict__ info,
-    const int *__restrict__ slice_base, const int64_t *__restrict__ slice_off,
-    const uint8_t *__restrict__ flags, const uint16_t *__restrict__ rownz,
-    const VT *__restrict__ pval, const uint16_t *__restrict__ pcol,
-    const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
-    double *__restrict__ partials) {
-  __shared__ __attribute__((aligned(16))) char lds[panel::kSegLdsBytes];
-  int p = blockIdx.x;
-  if (xcd_remap) {  // consecutive row panels on one XCD share their x window in its L2
-    const int per = gridDim.x >> 3;
-    p = (p & 7) * per + (p >> 3);
-  }
-  if (p >= npanels) return;  // uniform per workgroup
-  const PanelInfo pi = info[p];
-  panel::row_panel_seg<VT>(p, n, R, pi, slice_base, slice_off, flags, rownz, pval, pcol, x, y,
-                           alpha, beta, partials + 2 * p, lds);
 }
 
 }  // namespace
@@ -161,20 +141,18 @@ __global__ void k_panel_count(int npanels, int R, const PanelInfo *__restrict__ 
 template <int RPT>
 __global__ void k_panel_slice_scan(int nslices, int R, const uint32_t *__restrict__ cnt,
                                    uint32_t *__restrict__ srp, int64_t *__restrict__ snnz,
-                                   int *__restrict__ overflow, uint16_t *__restrict__ rownz,
-                                   int pad) {
+                                   int *__restrict__ overflow) {
   const int s = blockIdx.x;
   if (s >= nslices) return;
   typedef hipcub::BlockScan<uint32_t, 256> Scan;
   __shared__ typename Scan::TempStorage tmp;
   uint32_t v[RPT];
-  uint32_t local = 0, nz = 0;
+  uint32_t local = 0;
 #pragma unroll
   for (int j = 0; j < RPT; j++) {
     const int rl = threadIdx.x * RPT + j;
     v[j] = rl < R ? cnt[static_cast<int64_t>(s) * R + rl] : 0u;
     local += v[j];
-    nz += v[j] > 0 ? 1u : 0u;
   }
   uint32_t excl = 0, total = 0;
   Scan(tmp).ExclusiveSum(local, excl, total);
@@ -185,48 +163,11 @@ __global__ void k_panel_slice_scan(int nslices, int R, const uint32_t *__restric
     if (rl < R) srp[static_cast<int64_t>(s) * (R + 1) + rl] = run;
     run += v[j];
   }
-  if (rownz) {  // segmented form: the slice's rows with elements, in order
-    __syncthreads();
-    uint32_t zx = 0, zt = 0;
-    Scan(tmp).ExclusiveSum(nz, zx, zt);
-#pragma unroll
-    for (int j = 0; j < RPT; j++) {
-      const int rl = threadIdx.x * RPT + j;
-      if (v[j] > 0) rownz[static_cast<int64_t>(s) * R + zx++] = static_cast<uint16_t>(rl);
-    }
-  }
   if (threadIdx.x == 0) {
     srp[static_cast<int64_t>(s) * (R + 1) + R] = total;
-    snnz[s] = (total + pad - 1) / pad * pad;  // slices start aligned
+    snnz[s] = (total + kPanelPad - 1) / kPanelPad * kPanelPad;  // slices start aligned
     if (total > 65535u) overflow[0] = 1;
   }
-}
-
-// Segmented form: a flag bit per element that starts a row of its slice
-// (words of 32 elements; a slice starts on an 8-element boundary).
-__global__ void k_seg_flags(int nslices, int R, const uint32_t *__restrict__ srp,
-                            const int64_t *__restrict__ slice_off, uint32_t *__restrict__ flags) {
-  const int s = blockIdx.x;
-  if (s >= nslices) return;
-  const uint32_t *rps = srp + static_cast<int64_t>(s) * (R + 1);
-  for (int rl = threadIdx.x; rl < R; rl += blockDim.x) {
-    const uint32_t b = rps[rl], e = rps[rl + 1];
-    if (e > b) {
-      const int64_t g = slice_off[s] + b;
-      atomicOr(flags + (g >> 5), 1u << (g & 31));
-    }
-  }
-}
-
-// Position of the e-th element of a slice (row-major order) in the segmented
-// form: chunks of kSegChunk elements, thread t of a chunk holding elements
-// E*t .. E*t+E-1 (E = kSegItems), its pair i at 2*(i*T + t) (T = chunk
-// length / E).
-__device__ __forceinline__ int64_t seg_pos(int64_t e, int64_t len_pad) {
-  const int64_t c = e / kSegChunk, w = e % kSegChunk;
-  const int64_t T = min<int64_t>(kSegChunk, len_pad - c * kSegChunk) / kSegItems;
-  const int64_t th = w / kSegItems, in = w % kSegItems;
-  return c * kSegChunk + 2 * ((in >> 1) * T + th) + (in & 1);
 }
 
 __global__ void k_narrow_u16(int64_t count, const uint32_t *__restrict__ a,
@@ -242,7 +183,7 @@ __global__ void k_panel_scatter(int npanels, int R, const PanelInfo *__restrict_
                                 const int *__restrict__ slice_base,
                                 const int64_t *__restrict__ slice_off,
                                 const uint32_t *__restrict__ srp, VT *__restrict__ pval,
-                                uint16_t *__restrict__ pcol, int seg) {
+                                uint16_t *__restrict__ pcol) {
   const int p = blockIdx.x;
   if (p >= npanels) return;
   const PanelInfo pi = info[p];
@@ -251,20 +192,17 @@ __global__ void k_panel_scatter(int npanels, int R, const PanelInfo *__restrict_
   for (int rl = threadIdx.x; rl < pi.nrow; rl += blockDim.x) {
     const int r = pi.row0 + rl;
     int cur = -1;
-    int64_t pos = 0, sbase = 0, slen = 0;
+    int64_t pos = 0;
     for (int j = max(rp[r], w.x); j < min(rp[r + 1], w.y); j++) {
       const int c = col[j];
       const int q = c / kPanelCols;
       if (q != cur) {
         cur = q;
         const int s = sb + q - pi.q_lo;
-        sbase = slice_off[s];
-        slen = slice_off[s + 1] - sbase;
-        pos = srp[static_cast<int64_t>(s) * (R + 1) + rl];  // inside the slice
+        pos = slice_off[s] + srp[static_cast<int64_t>(s) * (R + 1) + rl];
       }
-      const int64_t at = sbase + (seg ? seg_pos(pos, slen) : pos);
-      pval[at] = val[j];
-      pcol[at] = static_cast<uint16_t>(c - q * kPanelCols);
+      pval[pos] = val[j];
+      pcol[pos] = static_cast<uint16_t>(c - q * kPanelCols);
       pos++;
     }
   }
@@ -274,19 +212,14 @@ __global__ void k_panel_scatter(int npanels, int R, const PanelInfo *__restrict_
 
 int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp, const int *col,
                 const void *val, const std::vector<ExecChunk> &ex, const int *h_rp,
-                bool force, int min_panels, bool seg, PanelFormat &f, bool *chosen,
-                hipStream_t s) {
+                bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s) {
   (void)n;
   (void)m;
   *chosen = false;
-  if (rows < 1 || rows > (seg ? kSegRowsMax : kPanelRowsMax)) {
-    set_error(SBLAS_ERR_ARGS, seg ? "panel format: rows per panel must be in [1, 1536]"
-                                  : "panel format: rows per panel must be in [1, 2048]");
+  if (rows < 1 || rows > kPanelRowsMax) {
+    set_error(SBLAS_ERR_ARGS, "panel format: rows per panel must be in [1, 2048]");
     return SBLAS_ERR_ARGS;
   }
-  f.seg = seg;
-  f.flags = nullptr;
-  f.rownz = nullptr;
   const int R = rows;
   f.rows = R;
   f.rpt = panel_rpt(R);
@@ -377,18 +310,16 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   f.slice_off = static_cast<int64_t *>(alloc(sizeof(int64_t) * (ns + 1)));
   int *overflow = static_cast<int *>(alloc(sizeof(int)));
   f.partials = static_cast<double *>(alloc(sizeof(double) * 2 * np));
-  if (seg) f.rownz = static_cast<uint16_t *>(alloc(sizeof(uint16_t) * ns * R));
-  if (!cnt || !f.srp32 || !snnz || !f.slice_off || !overflow || !f.partials ||
-      (seg && !f.rownz)) {
+  if (!cnt || !f.srp32 || !snnz || !f.slice_off || !overflow || !f.partials) {
     set_error(SBLAS_ERR_ALLOC, "panel format: allocation failed");
     return SBLAS_ERR_ALLOC;
   }
   // counts start at zero: written by a kernel (no memset nodes)
   const int64_t cnt_n = ns * R;
   hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((cnt_n + 255) / 256)), dim3(256), 0, s,
-                     cnt_n, cnt, 0u);
+                     cnt_n, cnt);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{1},
-                     reinterpret_cast<uint32_t *>(overflow), 0u);
+                     reinterpret_cast<uint32_t *>(overflow));
   if (f.npanels > 0)
     hipLaunchKernelGGL(k_panel_count, dim3(f.npanels), dim3(256), 0, s, f.npanels, R, f.info, win,
                        rp, col, f.slice_base, cnt);
@@ -396,7 +327,7 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
 #define SBLAS_SCAN_CASE(RPT)                                                                  \
   case RPT:                                                                                   \
     hipLaunchKernelGGL(k_panel_slice_scan<RPT>, dim3(f.nslices), dim3(256), 0, s, f.nslices, R, \
-                       cnt, f.srp32, snnz, overflow, f.rownz, seg ? kSegItems : kPanelPad);   \
+                       cnt, f.srp32, snnz, overflow);                                         \
     break;
     switch (f.rpt) {
       SBLAS_SCAN_CASE(1)
@@ -432,10 +363,10 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     }
     // snnz[nslices] must be 0 for the trailing total
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{2},
-                       reinterpret_cast<uint32_t *>(snnz + f.nslices), 0u);
+                       reinterpret_cast<uint32_t *>(snnz + f.nslices));
     SBLAS_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, snnz, f.slice_off, f.nslices + 1, s));
   }
-  // element arrays: slices padded to kPanelPad (kSegItems) elements, padding zeroed
+  // element arrays: slices padded to kPanelPad elements, padding zeroed
   int64_t padded = 0;
   if (f.nslices > 0) {
     SBLAS_HIP_TRY(hipMemcpyAsync(&padded, f.slice_off + f.nslices, sizeof(int64_t),
@@ -443,7 +374,7 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     SBLAS_HIP_TRY(hipStreamSynchronize(s));
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{2},
-                       reinterpret_cast<uint32_t *>(f.slice_off), 0u);
+                       reinterpret_cast<uint32_t *>(f.slice_off));
   }
   f.padded = padded;
   const size_t vbytes = val_type == SBLAS_VAL_F64 ? 8 : (val_type == SBLAS_VAL_F32 ? 4 : 2);
@@ -457,23 +388,9 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   {
     const int64_t wv = static_cast<int64_t>(vbytes) * cap / 4, wc = cap * 2 / 4;
     hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((wv + 255) / 256)), dim3(256), 0, s,
-                       wv, static_cast<uint32_t *>(f.pval), 0u);
-    // the segmented form marks padding with column 0xffff
+                       wv, static_cast<uint32_t *>(f.pval));
     hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((wc + 255) / 256)), dim3(256), 0, s,
-                       wc, reinterpret_cast<uint32_t *>(f.pcol), f.seg ? 0xffffffffu : 0u);
-  }
-  if (f.seg) {
-    const int64_t words = cap / 32 + 1;
-    f.flags = static_cast<uint8_t *>(alloc(sizeof(uint32_t) * words));
-    if (!f.flags) {
-      set_error(SBLAS_ERR_ALLOC, "panel format: allocation failed");
-      return SBLAS_ERR_ALLOC;
-    }
-    hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((words + 255) / 256)), dim3(256), 0,
-                       s, words, reinterpret_cast<uint32_t *>(f.flags), 0u);
-    if (f.nslices > 0)
-      hipLaunchKernelGGL(k_seg_flags, dim3(f.nslices), dim3(256), 0, s, f.nslices, R, f.srp32,
-                         f.slice_off, reinterpret_cast<uint32_t *>(f.flags));
+                       wc, reinterpret_cast<uint32_t *>(f.pcol));
   }
   if (f.npanels > 0) {
     const dim3 g(f.npanels), b(256);
@@ -481,17 +398,17 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
       case SBLAS_VAL_F64:
         hipLaunchKernelGGL(k_panel_scatter<double>, g, b, 0, s, f.npanels, R, f.info, win, rp, col,
                            static_cast<const double *>(val), f.slice_base, f.slice_off, f.srp32,
-                           static_cast<double *>(f.pval), f.pcol, f.seg ? 1 : 0);
+                           static_cast<double *>(f.pval), f.pcol);
         break;
       case SBLAS_VAL_F32:
         hipLaunchKernelGGL(k_panel_scatter<float>, g, b, 0, s, f.npanels, R, f.info, win, rp, col,
                            static_cast<const float *>(val), f.slice_base, f.slice_off, f.srp32,
-                           static_cast<float *>(f.pval), f.pcol, f.seg ? 1 : 0);
+                           static_cast<float *>(f.pval), f.pcol);
         break;
       default:
         hipLaunchKernelGGL(k_panel_scatter<uint16_t>, g, b, 0, s, f.npanels, R, f.info, win, rp,
                            col, static_cast<const uint16_t *>(val), f.slice_base, f.slice_off,
-                           f.srp32, static_cast<uint16_t *>(f.pval), f.pcol, f.seg ? 1 : 0);
+                           f.srp32, static_cast<uint16_t *>(f.pval), f.pcol);
         break;
     }
   }
@@ -501,13 +418,9 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   release(cnt);
   release(win);
   release(q_n);
-  if (f.srp || f.seg) {
+  if (f.srp) {
     release(f.srp32);
     f.srp32 = nullptr;
-  }
-  if (f.seg && f.srp) {  // the segmented kernel reads flags and row lists instead
-    release(f.srp);
-    f.srp = nullptr;
   }
   *chosen = true;
   return SBLAS_OK;
@@ -555,59 +468,30 @@ int panel_rpt(int rows) {
   return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : 8;
 }
 
-int panel_rows_for(int val_type, int m, int device, bool seg) {
+int panel_rows_for(int val_type, int m, int device) {
   int cus = 256, per_cu = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  const void *fn =
-      seg ? (val_type == SBLAS_VAL_F64   ? reinterpret_cast<const void *>(&k_panel_spmv_seg<double>)
-             : val_type == SBLAS_VAL_F32 ? reinterpret_cast<const void *>(&k_panel_spmv_seg<float>)
-                                         : reinterpret_cast<const void *>(&k_panel_spmv_seg<uint16_t>))
-          : (val_type == SBLAS_VAL_F64
-                 ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t>)
-             : val_type == SBLAS_VAL_F32
-                 ? reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t>)
-                 : reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t>));
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, seg ? kSegThreads : 256, 0) !=
-          hipSuccess ||
+  const void *fn = val_type == SBLAS_VAL_F64
+                       ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t>)
+                   : val_type == SBLAS_VAL_F32
+                       ? reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t>)
+                       : reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 2;
   const int64_t slots = static_cast<int64_t>(std::max(cus, 1)) * per_cu;
-  const int64_t cap = seg ? kSegRowsMax : kPanelRowsMax;
   // whole rounds of resident workgroups (at least two, so a slow workgroup is
   // not the whole tail), the tallest panels that fit
   for (int64_t k = 2;; k++) {
     int64_t R = (m + k * slots - 1) / (k * slots);
     R = (R + 63) / 64 * 64;
-    if (R <= cap) return static_cast<int>(std::max<int64_t>(R, 64));
+    if (R <= kPanelRowsMax) return static_cast<int>(std::max<int64_t>(R, 64));
   }
 }
-
-namespace {
-
-template <typename VT>
-int panel_seg_launch(const PanelFormat &f, int n, int xcd_remap, const double *x, double *y,
-                     double alpha, double beta, hipStream_t s) {
-  const int grid = xcd_remap ? ((f.npanels + 7) / 8) * 8 : f.npanels;
-  hipLaunchKernelGGL((k_panel_spmv_seg<VT>), dim3(grid), dim3(kSegThreads), 0, s, n, f.rows,
-                     f.npanels,
-                     xcd_remap, f.info, f.slice_base, f.slice_off, f.flags, f.rownz,
-                     static_cast<const VT *>(f.pval), f.pcol, x, y, alpha, beta, f.partials);
-  SBLAS_HIP_TRY(hipGetLastError());
-  return SBLAS_OK;
-}
-
-}  // namespace
 
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
                       const double *x, double *y, double alpha, double beta, hipStream_t s) {
   if (f.npanels <= 0) return SBLAS_OK;
-  if (f.seg) {
-    switch (val_type) {
-      case SBLAS_VAL_F64: return panel_seg_launch<double>(f, n, xcd_remap, x, y, alpha, beta, s);
-      case SBLAS_VAL_F32: return panel_seg_launch<float>(f, n, xcd_remap, x, y, alpha, beta, s);
-      default: return panel_seg_launch<uint16_t>(f, n, xcd_remap, x, y, alpha, beta, s);
-    }
-  }
   switch (val_type) {
     case SBLAS_VAL_F64: return panel_launch_vt<double>(f, m, n, xcd_remap, x, y, alpha, beta, s);
     case SBLAS_VAL_F32: return panel_launch_vt<float>(f, m, n, xcd_remap, x, y, alpha, beta, s);

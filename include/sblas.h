@@ -192,8 +192,6 @@ typedef struct sblas_opts {
   int xcd_remap;   /* 1: contiguous row ranges per XCD; 0: off; -1: auto   */
   int variant;     /* kernel 1 form: 0 auto, 1 sub-wave per row, 2 row blocks */
                    /* staged in LDS, 3 column panels with x staged in LDS    */
-                   /* (sums in the host loop's order), 4 column panels with  */
-                   /* segmented sums (fixed order, not the host loop's)      */
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
   int panel_rows;  /* column-panel form: rows per row panel (1..2048; 0 auto) */
   int reserved[5];
@@ -232,7 +230,7 @@ typedef struct sblas_plan_info {
   int64_t max_row;
   int launches;      /* kernel launches per sblas_plan_spmv                 */
   int variant;       /* kernel 1 form in use (1 sub-wave, 2 row blocks,     */
-                     /* 3 column panels, 4 segmented column panels)         */
+                     /* 3 column panels)                                    */
   int row_blocks;    /* row blocks of the row-block form                    */
   int items;         /* elements per thread of the row-block form           */
   int panel_rows;    /* column-panel form: rows per row panel               */
