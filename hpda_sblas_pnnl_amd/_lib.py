@@ -128,6 +128,16 @@ SIGNATURES = {
     "sblas_batch_get_choice": (_I, [_P, _I, _P]),
     "sblas_batch_get_shape": (_I, [_P, _P, _P, _P]),
     "sblas_batch_destroy": (_I, [_P]),
+    "sortCOORow": (None, [_I, _I, _I, _P, _P, _P]),
+    "sortCOOCol": (None, [_I, _I, _I, _P, _P, _P]),
+    "coo2csr": (None, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "coo2csc": (None, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "sblas_coo_sort": (_I, [_I, _I, _I64, _P, _P, _P, _I, _I, _P]),
+    "sblas_coo_ptr": (_I, [_I, _I64, _P, _P, _P]),
+    "sblas_mm_read_coo": (_I, [ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                               ctypes.POINTER(_I), ctypes.POINTER(_P), ctypes.POINTER(_P),
+                               ctypes.POINTER(_P), ctypes.POINTER(_I)]),
+    "sblas_free": (None, [_P]),
 }
 
 

@@ -29,7 +29,10 @@ HIP_SOURCES = [
     "sblas_csr5.hip",
     "sblas_batch.hip",
     "sblas_panel.hip",
+    "sblas_convert.hip",
     "batch.cpp",
+    "ingest.cpp",
+    "mmread.cpp",
     "plan.cpp",
     "dropin.cpp",
     "partition.cpp",

@@ -20,6 +20,7 @@
 #include <limits>
 #include <vector>
 
+#include "host_util.hpp"
 #include "plan.hpp"
 
 namespace sblas {
@@ -39,33 +40,6 @@ int env_int(const char *name, int fallback) {
   const char *e = std::getenv(name);
   return (e && *e) ? std::atoi(e) : fallback;
 }
-
-// Restores the caller's current device on scope exit.
-struct DeviceGuard {
-  int prev = -1;
-  bool ok = false;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) return;
-    ok = hipSetDevice(dev) == hipSuccess;
-  }
-  ~DeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
-struct DeviceArrays {
-  std::vector<void *> ptrs;
-  ~DeviceArrays() {
-    for (void *p : ptrs) (void)hipFree(p);
-  }
-  template <typename T>
-  T *alloc(size_t count) {
-    void *p = nullptr;
-    if (hipMalloc(&p, sizeof(T) * (count ? count : 1)) != hipSuccess) return nullptr;
-    ptrs.push_back(p);
-    return static_cast<T *>(p);
-  }
-};
 
 struct EventPair {
   hipEvent_t a = nullptr, b = nullptr;

@@ -1,0 +1,56 @@
+// host_util.hpp -- host-side helpers of the synchronous entry points that
+// take host arrays (the reference's drop-in calls and ingestion helpers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+namespace sblas {
+
+// Makes `dev` current for the scope and restores the caller's device after.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = false;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) return;
+    ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+  DeviceGuard(const DeviceGuard &) = delete;
+  DeviceGuard &operator=(const DeviceGuard &) = delete;
+};
+
+// Device allocations freed when the scope ends.
+struct DeviceArrays {
+  std::vector<void *> ptrs;
+  DeviceArrays() = default;
+  DeviceArrays(const DeviceArrays &) = delete;
+  DeviceArrays &operator=(const DeviceArrays &) = delete;
+  ~DeviceArrays() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T *alloc(size_t count) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * (count ? count : 1)) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return static_cast<T *>(p);
+  }
+};
+
+// A non-blocking stream destroyed when the scope ends (nullptr if creation failed).
+struct OwnedStream {
+  hipStream_t s = nullptr;
+  OwnedStream() { (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+  ~OwnedStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  OwnedStream(const OwnedStream &) = delete;
+  OwnedStream &operator=(const OwnedStream &) = delete;
+};
+
+}  // namespace sblas

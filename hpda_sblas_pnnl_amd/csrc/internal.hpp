@@ -107,6 +107,14 @@ int check_indices(const int *a, int64_t count, int lo, int hi, bool sorted, hipS
 int csc_to_csr_device(int m, int n, int nnz, const int *cp, const int *ri, const void *val,
                       int val_type, int *rp, int *col, void *val_out, hipStream_t s);
 
+// matrix ingestion (sblas_convert.hip).  Synchronous on s; scratch allocated
+// per call.  Sorts (row, col, val) in place by (row, col) or, with by_col, by
+// (col, row); equal pairs keep their input order.  Indices are validated.
+int coo_sort_device(int m, int n, int64_t nnz, int *row, int *col, void *val, int val_type,
+                    bool by_col, hipStream_t s);
+// ptr[count + 1]: ptr[r] = first position of r in the sorted idx[nnz].
+int ptr_from_sorted_device(int count, int64_t nnz, const int *idx, int *ptr, hipStream_t s);
+
 // segmented tiles (sblas_tiles.hip): merge-path CSR and COO
 constexpr int kTileBlock = 256;   // threads per tile block
 constexpr int kTileItems = 8;     // merge items (or COO elements) per thread per tile

@@ -191,3 +191,37 @@ class SpmvPlan:
             self.close()
         except Exception:  # noqa: BLE001  (interpreter shutdown)
             pass
+
+
+def _int32_dev(t, name):
+    torch = _torch()
+    if not (t.is_cuda and t.dtype == torch.int32 and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous int32 device tensor")
+    return t
+
+
+def coo_sort(row, col, val, m, n, by_col=False, stream=None):
+    """Sorts device COO tensors in place by (row, column), or by (column, row)
+    with ``by_col`` -- cusparseXcoosortByRow / ByColumn with the value gather
+    (src/spmv_helper.cu:406-476).  Stable: equal pairs keep their order.
+    Returns when the work on ``stream`` is done."""
+    _int32_dev(row, "row"), _int32_dev(col, "col")
+    if not (val.is_cuda and val.is_contiguous()):
+        raise TypeError("val must be a contiguous device tensor")
+    nnz = row.numel()
+    if col.numel() != nnz or val.numel() != nnz:
+        raise ValueError("row, col and val must have the same length")
+    check(lib().sblas_coo_sort(m, n, nnz, row.data_ptr(), col.data_ptr(), val.data_ptr(),
+                               val_type_of(val), 1 if by_col else 0, _stream_ptr(stream)),
+          "sblas_coo_sort")
+
+
+def coo_ptr(sorted_idx, count, stream=None):
+    """Compressed pointers of a sorted index tensor (cusparseXcoo2csr,
+    src/spmv_helper.cu:478-491): an int32 tensor of count + 1 entries."""
+    torch = _torch()
+    _int32_dev(sorted_idx, "sorted_idx")
+    ptr = torch.empty(count + 1, dtype=torch.int32, device=sorted_idx.device)
+    check(lib().sblas_coo_ptr(count, sorted_idx.numel(), sorted_idx.data_ptr(), ptr.data_ptr(),
+                              _stream_ptr(stream)), "sblas_coo_ptr")
+    return ptr

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -22,7 +23,8 @@ __all__ = [
     "spMV_mgpu_baseline", "spMV_mgpu_baseline_csc", "spMV_mgpu_baseline_coo", "spMV_mgpu_v1",
     "spMV_mgpu_v1_numa", "spMV_mgpu_v1_numa_csc", "spMV_mgpu_v1_numa_coo", "spMV_mgpu_v2",
     "spMspV_mgpu_v1", "spMspV_mgpu_v1_numa", "spMspV_mgpu_v2", "get_row_from_index",
-    "findFirstInSorted", "findLastInSorted",
+    "findFirstInSorted", "findLastInSorted", "sortCOORow", "sortCOOCol", "coo2csr", "coo2csc",
+    "mm_read_coo",
 ]
 
 
@@ -212,3 +214,86 @@ def partition(mode: str, *, C: int, m: int = 0, n: int = 0, nnz: int = 0, row_pt
         got = L.sblas_partition_v2(m, rp.ctypes.data, nnz, nb, q, out.ctypes.data, T)
         return (0 if got >= 0 else -got), out[:T]
     raise ValueError(f"unknown partition mode {mode!r}")
+
+
+# --------------------------------------------------------------------------
+# matrix ingestion (src/spmv_helper.cu:277-642, include/mmio.h)
+# --------------------------------------------------------------------------
+
+def _inplace(a, dtype, name):
+    if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.flags.c_contiguous):
+        raise TypeError(f"{name} must be a C-contiguous {np.dtype(dtype).name} array (sorted in place)")
+    return a
+
+
+def _void_call(name, *args):
+    getattr(lib(), name)(*args)
+    st = lib().sblas_last_error()
+    if st != 0:
+        raise SblasError(st, name)
+
+
+def _sort_coo(name, m, n, nnz, cooVal, cooRowIdx, cooColIdx):
+    v = _inplace(cooVal, np.float64, "cooVal")
+    r = _inplace(cooRowIdx, np.int32, "cooRowIdx")
+    c = _inplace(cooColIdx, np.int32, "cooColIdx")
+    if min(len(v), len(r), len(c)) < nnz:
+        raise ValueError("arrays shorter than nnz")
+    _void_call(name, m, n, nnz, v.ctypes.data, r.ctypes.data, c.ctypes.data)
+
+
+def sortCOORow(m, n, nnz, cooVal, cooRowIdx, cooColIdx):
+    """Sorts the COO arrays in place by row, then column (on the device)."""
+    _sort_coo("sortCOORow", m, n, nnz, cooVal, cooRowIdx, cooColIdx)
+
+
+def sortCOOCol(m, n, nnz, cooVal, cooRowIdx, cooColIdx):
+    """Sorts the COO arrays in place by column, then row (on the device)."""
+    _sort_coo("sortCOOCol", m, n, nnz, cooVal, cooRowIdx, cooColIdx)
+
+
+def _convert(name, m, n, nnz, cooVal, cooRowIdx, cooColIdx, major):
+    v, r, c = _f64(cooVal), _i32(cooRowIdx), _i32(cooColIdx)
+    if min(len(v), len(r), len(c)) < nnz:
+        raise ValueError("arrays shorter than nnz")
+    out_v = np.empty(max(nnz, 0), np.float64)
+    out_p = np.empty(max(major, 0) + 1, np.int32)
+    out_i = np.empty(max(nnz, 0), np.int32)
+    _void_call(name, m, n, nnz, v.ctypes.data, r.ctypes.data, c.ctypes.data, out_v.ctypes.data,
+               out_p.ctypes.data, out_i.ctypes.data)
+    return out_v, out_p, out_i
+
+
+def coo2csr(m, n, nnz, cooVal, cooRowIdx, cooColIdx):
+    """COO (any order) -> (csrVal, csrRowPtr, csrColIdx); the input is not modified."""
+    return _convert("coo2csr", m, n, nnz, cooVal, cooRowIdx, cooColIdx, m)
+
+
+def coo2csc(m, n, nnz, cooVal, cooRowIdx, cooColIdx):
+    """COO (any order) -> (cscVal, cscColPtr, cscRowIdx); the input is not modified."""
+    return _convert("coo2csc", m, n, nnz, cooVal, cooRowIdx, cooColIdx, n)
+
+
+SYMMETRY = ("general", "symmetric", "skew-symmetric", "hermitian")
+
+
+def mm_read_coo(path):
+    """A Matrix Market coordinate file as COO: (m, n, row_idx, col_idx, val,
+    symmetry) with 0-based int32 indices in file order, float64 values and
+    the symmetry name (stored entries only, not expanded)."""
+    m, n, nnz, sym = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    pr, pc, pv = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    st = lib().sblas_mm_read_coo(os.fsencode(path), ctypes.byref(m), ctypes.byref(n),
+                                 ctypes.byref(nnz), ctypes.byref(pr), ctypes.byref(pc),
+                                 ctypes.byref(pv), ctypes.byref(sym))
+    if st != 0:
+        raise SblasError(st, f"mm_read_coo({path})")
+    try:
+        k = nnz.value
+        row = np.ctypeslib.as_array(ctypes.cast(pr, ctypes.POINTER(ctypes.c_int)), (max(k, 1),))[:k].copy()
+        col = np.ctypeslib.as_array(ctypes.cast(pc, ctypes.POINTER(ctypes.c_int)), (max(k, 1),))[:k].copy()
+        val = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_double)), (max(k, 1),))[:k].copy()
+    finally:
+        for p in (pr, pc, pv):
+            lib().sblas_free(p)
+    return m.value, n.value, row, col, val, SYMMETRY[sym.value]

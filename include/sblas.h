@@ -1,7 +1,7 @@
 /*
  * sblas.h -- C ABI of the MI355X SpMV engine (libsblas.so).
  *
- * Three layers:
+ * Four layers:
  *   1. Drop-in entry points with the reference's names and argument lists
  *      (include/spmv_kernel.h:211-296 of the reference).  Host pointers,
  *      synchronous, y = alpha*A*x + beta*y, ngpu must be 1.
@@ -10,6 +10,8 @@
  *   3. Plan API on device-resident arrays (the analogue of the reference's
  *      CSR5 handle, include/anonymouslib_cuda.h:11-53): analyse once, run
  *      many times on a caller stream.  Also the batched multi-matrix form.
+ *   4. Matrix ingestion: the reference's COO sort / conversion helpers (on
+ *      the device) and a Matrix Market reader.
  *
  * All functions are thread-safe.  Errors never exit the process: the
  * functions return a status (or NaN timings for the spmv_ret entry points)
@@ -291,6 +293,51 @@ int sblas_batch_get_choice(sblas_batch batch, int index, sblas_batch_choice *out
 /* work items, persistent grid size, kernel launches per run */
 int sblas_batch_get_shape(sblas_batch batch, int *work_items, int *grid, int *launches);
 int sblas_batch_destroy(sblas_batch batch);
+
+/* ---------------------------------------------------------------------- */
+/* 4. matrix ingestion                                                     */
+/* ---------------------------------------------------------------------- */
+/* The reference's host helpers, same names and arguments.  Host arrays,
+ * synchronous, the work done on the device.  The reference returns nothing
+ * and exits on errors; these return nothing, leave sblas_last_error() ==
+ * SBLAS_OK on success, and leave the arrays untouched on error (indices are
+ * validated first: rows in [0, m), columns in [0, n)).  Sorts are stable:
+ * elements with equal (row, column) keep their input order. */
+/* src/spmv_helper.cu:277-331 -- in place, by row, then column */
+void sortCOORow(int m, int n, int nnz, double *cooVal, int *cooRowIdx, int *cooColIdx);
+/* src/spmv_helper.cu:333-387 -- in place, by column, then row */
+void sortCOOCol(int m, int n, int nnz, double *cooVal, int *cooRowIdx, int *cooColIdx);
+/* src/spmv_helper.cu:493-573 -- COO in any order -> CSR (csrRowPtr[m+1],
+ * columns ascending within rows); the COO input is not modified */
+void coo2csr(int m, int n, int nnz, double *cooVal, int *cooRowIdx, int *cooColIdx,
+             double *csrVal, int *csrRowPtr, int *csrColIdx);
+/* src/spmv_helper.cu:575-642 -- COO in any order -> CSC (cscColPtr[n+1]) */
+void coo2csc(int m, int n, int nnz, double *cooVal, int *cooRowIdx, int *cooColIdx,
+             double *cscVal, int *cscColPtr, int *cscRowIdx);
+
+/* Device forms (cusparseXcoosortByRow / ByColumn with the value gather,
+ * src/spmv_helper.cu:406-476, and cusparseXcoo2csr, :478-491).  Device
+ * arrays; the call returns when the work on `stream` is done; scratch is
+ * allocated per call.  sblas_coo_sort sorts in place by (row, column), or by
+ * (column, row) with by_col != 0; d_val holds val_type values. */
+int sblas_coo_sort(int m, int n, int64_t nnz, int *d_row_idx, int *d_col_idx, void *d_val,
+                   int val_type, int by_col, void *stream);
+/* d_ptr[count+1]: d_ptr[r] = position of the first index >= r in the sorted
+ * d_sorted_idx[nnz] (row pointers of row-sorted COO, or column pointers of
+ * column-sorted COO). */
+int sblas_coo_ptr(int count, int64_t nnz, const int *d_sorted_idx, int *d_ptr, void *stream);
+
+/* include/mmio.h:254-367 and test/dspmv_test_dgx1.cu:133-199 -- a Matrix
+ * Market coordinate file as COO: real, integer (stored as double) or pattern
+ * (1.0) fields, 0-based indices, entries in file order; symmetric,
+ * skew-symmetric and Hermitian files give their stored entries, not expanded
+ * (*symmetry: 0 general, 1 symmetric, 2 skew-symmetric, 3 Hermitian; may be
+ * NULL).  The arrays come from malloc: release them with sblas_free.  Complex
+ * or dense (array) files: SBLAS_ERR_UNSUPPORTED; unreadable or malformed
+ * files and out-of-range indices: SBLAS_ERR_ARGS. */
+int sblas_mm_read_coo(const char *path, int *m, int *n, int *nnz, int **row_idx, int **col_idx,
+                      double **val, int *symmetry);
+void sblas_free(void *p);
 
 #ifdef __cplusplus
 }

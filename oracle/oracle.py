@@ -272,6 +272,36 @@ def gen_g(n):
     return r, c, v
 
 
+def coo_sort(row, col, val, by_col=False):
+    """COO sorted by (row, column), or by (column, row) with by_col, equal
+    pairs in input order: the order cusparseXcoosortByRow / ByColumn plus the
+    value gather leave (src/spmv_helper.cu:277-387; the tie order of equal
+    keys is the stable one, cuSPARSE's own is not published).  Returns new
+    (row, col, val) arrays."""
+    row, col, val = np.asarray(row), np.asarray(col), np.asarray(val)
+    order = np.lexsort((row, col)) if by_col else np.lexsort((col, row))  # stable
+    return row[order], col[order], val[order]
+
+
+def coo_ptr(count, sorted_idx):
+    """Compressed pointer array of a sorted index array (cusparseXcoo2csr,
+    src/spmv_helper.cu:478-491): ptr[r] = first position of an index >= r."""
+    return np.searchsorted(np.asarray(sorted_idx), np.arange(count + 1), side="left").astype(
+        np.int32)
+
+
+def coo_to_csr(m, n, row, col, val):
+    """coo2csr (src/spmv_helper.cu:493-573): (csrVal, csrRowPtr, csrColIdx)."""
+    r, c, v = coo_sort(row, col, val)
+    return v, coo_ptr(m, r), c.astype(np.int32)
+
+
+def coo_to_csc(m, n, row, col, val):
+    """coo2csc (src/spmv_helper.cu:575-642): (cscVal, cscColPtr, cscRowIdx)."""
+    r, c, v = coo_sort(row, col, val, by_col=True)
+    return v, coo_ptr(n, c), r.astype(np.int32)
+
+
 def err_normwise(y, r) -> float:
     y, r = _f64(y), _f64(r)
     return lib().ora_err_normwise(len(y), _p(y), _p(r))

@@ -13,6 +13,18 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_device_first(request):
+    """torch carries its own HIP runtime beside the system one libsblas links.
+    When both share a process, torch must open the device first: if libsblas
+    initialises the GPU before torch does, torch then finds no GPU."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda")
+    yield
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import oracle as ora
