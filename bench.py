@@ -124,6 +124,47 @@ def _time_calls(fn, reps=20, warmup=3):
     return float(np.median([a.elapsed_time(b) for a, b in ts]))
 
 
+def extras_ingest(A, rp, col, val, log_to):
+    """Matrix ingestion on the device (sortCOORow + coo2csr's pointer step,
+    src/spmv_helper.cu:277-573): the R-MAT matrix as COO in a random order,
+    sorted back by (row, column) and compressed; checked against the
+    generator's CSR.  Wall time of the synchronous calls (index validation and
+    per-call scratch included).  CPU side: numpy's stable lexsort on a
+    4M-element sample, one thread."""
+    import torch
+    from hpda_sblas_pnnl_amd.device import coo_ptr, coo_sort
+    row = torch.repeat_interleave(torch.arange(A.m, dtype=torch.int32, device=rp.device),
+                                  torch.diff(rp).long())
+    g = torch.Generator(device=rp.device)
+    g.manual_seed(5)
+    perm = torch.randperm(A.nnz, device=rp.device, generator=g)
+    times = []
+    for _ in range(3):
+        r, c, v = row[perm].contiguous(), col[perm].contiguous(), val[perm].contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        coo_sort(r, c, v, A.m, A.n)
+        t1 = time.perf_counter()
+        ptr = coo_ptr(r, A.m)
+        t2 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1))
+    exact = bool(torch.equal(r, row) and torch.equal(c, col) and torch.equal(v, val) and
+                 torch.equal(ptr, rp))
+    ts, tp = min(times)
+    k = min(A.nnz, 4_000_000)
+    rr, cc = row[perm[:k]].cpu().numpy(), col[perm[:k]].cpu().numpy()
+    t0 = time.perf_counter()
+    np.lexsort((cc, rr))
+    t_cpu = time.perf_counter() - t0
+    out = {"nnz": A.nnz, "sort_ms": round(ts * 1e3, 2), "ptr_ms": round(tp * 1e3, 3),
+           "Melem_per_s": round(A.nnz / (ts + tp) / 1e6, 1), "exact": exact,
+           "cpu_sample": {"nnz": k, "ms": round(t_cpu * 1e3, 1),
+                          "Melem_per_s": round(k / t_cpu / 1e6, 1), "cores": 1,
+                          "kind": "numpy lexsort (stable)"}}
+    log_to(f"[bench] ingest: {out}")
+    return out
+
+
 def extras_rmat(log_to):
     """R-MAT scale 22 (edge factor 17, binary values, alpha 1, beta 0.5): FP64
     kernels 1/2/3, COO and CSC forms; FP32 / BF16 values and a k=8 block of
@@ -177,6 +218,7 @@ def extras_rmat(log_to):
     del p, dcp, drows, dcv
     out["fp64"] = kernels
     base_ms = kernels["kernel1_rowblock"]["ms"]
+    out["ingest"] = extras_ingest(A, rp, col, val, log_to)
 
     prec = {}
     v32 = torch.from_numpy(A.val.astype(np.float32)).to(dev)
