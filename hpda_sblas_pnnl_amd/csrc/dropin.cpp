@@ -61,9 +61,12 @@ struct EventPair {
 // Core of every entry point: host arrays in, y out.  format CSR: a0 = row
 // pointers (m+1), a1 = columns; COO: a0 = row indices (nnz), a1 = columns;
 // CSC: a0 = column pointers (n+1), a1 = row indices.
+// With sparse_x (CSR only), elements whose x[col] == 0 are dropped on the
+// device before the plan is built (spMspV).
 spmv_ret run_format(int format, int m, int n, int nnz, const double *alpha, const double *val,
                     const int *a0, const int *a1, const double *x, const double *beta, double *y,
-                    int kernel, int part_mode, int nb, int q, double host_prep_s) {
+                    int kernel, int part_mode, int nb, int q, double host_prep_s,
+                    bool sparse_x = false) {
   spmv_ret ret{0, 0, 0, 0, 0};
   DeviceGuard guard(env_int("SBLAS_DEVICE", 0));
   if (!guard.ok) {
@@ -107,6 +110,27 @@ spmv_ret run_format(int format, int m, int n, int nnz, const double *alpha, cons
     set_error(SBLAS_ERR_HIP, "upload failed");
     return nan_ret();
   }
+  int64_t nz = nnz;
+  if (sparse_x) {
+    const double t_f0 = now_s();
+    int *rp2 = mem.alloc<int>(static_cast<size_t>(m) + 1), *c2 = mem.alloc<int>(nnz);
+    double *v2 = mem.alloc<double>(nnz);
+    if (!rp2 || !c2 || !v2) {
+      set_error(SBLAS_ERR_ALLOC, "device allocation failed");
+      return nan_ret();
+    }
+    if (csr_drop_zero_x_device(m, n, nnz, d_a0, d_col, d_val, d_x, rp2, c2, v2, &nz, s) !=
+        SBLAS_OK)
+      return nan_ret();
+    d_a0 = rp2;
+    d_col = c2;
+    d_val = v2;
+    host_prep_s += now_s() - t_f0;
+    if (part_mode == SBLAS_PART_V2) {  // the task count follows the kept elements
+      const int T = sblas::v2_num_tasks(static_cast<int>(nz), nb, q);
+      q = (T > 0 && T <= 64) ? q : 1;
+    }
+  }
   sblas_opts o;
   sblas_opts_default(&o);
   o.kernel = kernel;
@@ -117,10 +141,10 @@ spmv_ret run_format(int format, int m, int n, int nnz, const double *alpha, cons
   sblas_plan plan = nullptr;
   const double t_plan0 = now_s();
   int st = format == SBLAS_FMT_CSR
-               ? sblas_plan_create_csr(m, n, nnz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan)
+               ? sblas_plan_create_csr(m, n, nz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan)
            : format == SBLAS_FMT_COO
-               ? sblas_plan_create_coo(m, n, nnz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan)
-               : sblas_plan_create_csc(m, n, nnz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan);
+               ? sblas_plan_create_coo(m, n, nz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan)
+               : sblas_plan_create_csc(m, n, nz, d_a0, d_col, d_val, SBLAS_VAL_F64, &o, &plan);
   if (st != SBLAS_OK) return nan_ret();
   struct PlanGuard {
     sblas_plan p;
@@ -258,37 +282,16 @@ spmv_ret spMV_mgpu_v2(int m, int n, int nnz, double *alpha, double *csrVal, int 
 }
 
 // x-sparse forms: entries whose x[col] == 0 are dropped first
-// (src/dspmv_mgpu_v1.cu:19-74, src/dspmv_mgpu_v2.cu:35-101).
-static bool drop_zero_x(int m, int nnz, const double *val, const int *rp, const int *col,
-                        const double *x, std::vector<double> &v2, std::vector<int> &rp2,
-                        std::vector<int> &c2) {
-  (void)nnz;
-  rp2.assign(static_cast<size_t>(m) + 1, 0);
-  v2.clear();
-  c2.clear();
-  for (int i = 0; i < m; i++) {
-    for (int j = rp[i]; j < rp[i + 1]; j++)
-      if (x[col[j]] != 0.0) {
-        v2.push_back(val[j]);
-        c2.push_back(col[j]);
-      }
-    rp2[i + 1] = static_cast<int>(v2.size());
-  }
-  return true;
-}
-
+// (src/dspmv_mgpu_v1.cu:19-74, src/dspmv_mgpu_v2.cu:35-101), here on the
+// device after the upload; the filter's time counts in part_time.
 spmv_ret spMspV_mgpu_v1(int m, int n, int nnz, double *alpha, double *csrVal, int *csrRowPtr,
                         int *csrColIndex, double *x, double *beta, double *y, int ngpu,
                         int kernel) {
   if (!check_common(m, n, nnz, alpha, beta, csrVal, csrRowPtr, csrColIndex, x, y, ngpu) ||
       !check_kernel(kernel) || !check_csr(m, nnz, csrRowPtr))
     return nan_ret();
-  const double t0 = now_s();
-  std::vector<double> v2;
-  std::vector<int> rp2, c2;
-  drop_zero_x(m, nnz, csrVal, csrRowPtr, csrColIndex, x, v2, rp2, c2);
-  return run_csr(m, n, static_cast<int>(v2.size()), alpha, v2.data(), rp2.data(), c2.data(), x,
-                 beta, y, kernel, SBLAS_PART_V1, 0, 1, now_s() - t0);
+  return run_format(SBLAS_FMT_CSR, m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y,
+                    kernel, SBLAS_PART_V1, 0, 1, 0.0, true);
 }
 
 spmv_ret spMspV_mgpu_v1_numa(int m, int n, int nnz, double *alpha, double *csrVal,
@@ -312,15 +315,8 @@ spmv_ret spMspV_mgpu_v2(int m, int n, int nnz, double *alpha, double *csrVal, in
     set_error(SBLAS_ERR_ARGS, "nb and copy_of_workspace must be >= 1");
     return nan_ret();
   }
-  const double t0 = now_s();
-  std::vector<double> v2;
-  std::vector<int> rp2, c2;
-  drop_zero_x(m, nnz, csrVal, csrRowPtr, csrColIndex, x, v2, rp2, c2);
-  const int nz = static_cast<int>(v2.size());
-  const int T = sblas::v2_num_tasks(nz, nb, copy_of_workspace);
-  const int q = (T > 0 && T <= 64) ? copy_of_workspace : 1;
-  return run_csr(m, n, nz, alpha, v2.data(), rp2.data(), c2.data(), x, beta, y, kernel,
-                 SBLAS_PART_V2, nb, q, now_s() - t0);
+  return run_format(SBLAS_FMT_CSR, m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y,
+                    kernel, SBLAS_PART_V2, nb, copy_of_workspace, 0.0, true);
 }
 
 spmv_ret spMV_mgpu_baseline_coo(int m, int n, int nnz, double *alpha, double *cooVal,

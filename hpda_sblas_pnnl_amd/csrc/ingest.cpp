@@ -1,6 +1,7 @@
 // ingest.cpp -- C entry points of matrix ingestion: the reference's host
 // helpers sortCOORow / sortCOOCol / coo2csr / coo2csc (src/spmv_helper.cu:
-// 277-642) and their device-array forms.
+// 277-642), their device-array forms, and spMspV's sparse-x filter on device
+// arrays.
 //
 // Like the reference's, the host helpers upload the caller's arrays, work on
 // the device and copy the result back, synchronously.  sortCOORow / sortCOOCol
@@ -138,6 +139,15 @@ int sblas_coo_sort(int m, int n, int64_t nnz, int *d_row_idx, int *d_col_idx, vo
 int sblas_coo_ptr(int count, int64_t nnz, const int *d_sorted_idx, int *d_ptr, void *stream) {
   sblas::clear_error();
   return sblas::ptr_from_sorted_device(count, nnz, d_sorted_idx, d_ptr,
+                                       static_cast<hipStream_t>(stream));
+}
+
+int sblas_csr_drop_zero_x(int m, int n, int64_t nnz, const int *d_row_ptr, const int *d_col,
+                          const double *d_val, const double *d_x, int *d_row_ptr_out,
+                          int *d_col_out, double *d_val_out, int64_t *nnz_out, void *stream) {
+  sblas::clear_error();
+  return sblas::csr_drop_zero_x_device(m, n, nnz, d_row_ptr, d_col, d_val, d_x, d_row_ptr_out,
+                                       d_col_out, d_val_out, nnz_out,
                                        static_cast<hipStream_t>(stream));
 }
 

@@ -112,6 +112,12 @@ int csc_to_csr_device(int m, int n, int nnz, const int *cp, const int *ri, const
 // (col, row); equal pairs keep their input order.  Indices are validated.
 int coo_sort_device(int m, int n, int64_t nnz, int *row, int *col, void *val, int val_type,
                     bool by_col, hipStream_t s);
+// spMspV's filter: drops the elements whose x[col] == 0 (rows keep their
+// order); outputs hold at least nnz elements; *nnz_out = elements kept.
+// Synchronous on s.
+int csr_drop_zero_x_device(int m, int n, int64_t nnz, const int *rp, const int *col,
+                           const double *val, const double *x, int *rp_out, int *col_out,
+                           double *val_out, int64_t *nnz_out, hipStream_t s);
 // ptr[count + 1]: ptr[r] = first position of r in the sorted idx[nnz].
 int ptr_from_sorted_device(int count, int64_t nnz, const int *idx, int *ptr, hipStream_t s);
 

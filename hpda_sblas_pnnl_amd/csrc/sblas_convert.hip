@@ -12,6 +12,13 @@
 // covers only the bits the two indices need.  Pointers are the first position
 // of every row in the sorted index array, one binary search per row.  All of
 // it is integer work and data movement: values are copied, never combined.
+//
+// Also here: the sparse-x filter of spMspV (src/dspmv_mgpu_v1.cu:19-74,
+// src/dspmv_mgpu_v2.cu:35-101), which the reference runs as a host loop
+// before uploading: elements whose x[col] is zero are dropped, row by row in
+// order.  On the device: a keep flag per element, an exclusive scan of the
+// flags, a scatter of the kept elements, and new row pointers read off the
+// scan at the old ones.
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
@@ -58,6 +65,8 @@ __global__ void k_ptr_from_sorted(int count, int nnz, const int *__restrict__ id
   ptr[r] = lo;
 }
 
+__global__ void k_fill_zero_i32(int *__restrict__ p) { *p = 0; }
+
 template <typename VT>
 int sort_pairs(int nnz, int end_bit, uint64_t *keys_in, uint64_t *keys_out, VT *val,
                VT *val_out, hipStream_t s) {
@@ -86,7 +95,97 @@ int sort_pairs(int nnz, int end_bit, uint64_t *keys_in, uint64_t *keys_out, VT *
   return SBLAS_OK;
 }
 
+// element kept by the sparse-x filter: x at its column is not zero
+__global__ void k_keep_flags(int nnz, const int *__restrict__ col, const double *__restrict__ x,
+                             int *__restrict__ keep) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  keep[j] = x[col[j]] != 0.0 ? 1 : 0;
+}
+
+__global__ void k_scatter_kept(int nnz, const int *__restrict__ keep, const int *__restrict__ pos,
+                               const int *__restrict__ col, const double *__restrict__ val,
+                               int *__restrict__ col_out, double *__restrict__ val_out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz || !keep[j]) return;
+  col_out[pos[j]] = col[j];
+  val_out[pos[j]] = val[j];
+}
+
+// rp_out[r] = kept elements before rp[r]; pos has nnz + 1 entries (the last
+// one is the kept total)
+__global__ void k_rebase_rows(int m, const int *__restrict__ rp, const int *__restrict__ pos,
+                              int *__restrict__ rp_out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > m) return;
+  rp_out[r] = pos[rp[r]];
+}
+
 }  // namespace
+
+int csr_drop_zero_x_device(int m, int n, int64_t nnz, const int *rp, const int *col,
+                           const double *val, const double *x, int *rp_out, int *col_out,
+                           double *val_out, int64_t *nnz_out, hipStream_t s) {
+  if (m < 0 || n < 0 || nnz < 0 || nnz > INT_MAX - 1 || !rp || !rp_out || !nnz_out ||
+      (nnz > 0 && (!col || !val || !x || !col_out || !val_out))) {
+    set_error(SBLAS_ERR_ARGS, "sparse-x filter: bad sizes or pointers");
+    return SBLAS_ERR_ARGS;
+  }
+  bool ok = false;
+  int st = check_indices(rp, static_cast<int64_t>(m) + 1, 0, static_cast<int>(nnz) + 1, true, s,
+                         &ok);
+  if (st != SBLAS_OK) return st;
+  if (ok && nnz > 0 && (st = check_indices(col, nnz, 0, n, false, s, &ok)) != SBLAS_OK) return st;
+  if (!ok) {
+    set_error(SBLAS_ERR_ARGS, "sparse-x filter: row pointers or column indices out of range");
+    return SBLAS_ERR_ARGS;
+  }
+  const int count = static_cast<int>(nnz);
+  int *keep = nullptr, *pos = nullptr;
+  void *temp = nullptr;
+  size_t temp_bytes = 0;
+  auto release = [&] {
+    if (keep) (void)hipFree(keep);
+    if (pos) (void)hipFree(pos);
+    if (temp) (void)hipFree(temp);
+  };
+  // keep[nnz] = 0 so the scan's last entry is the kept total
+  if (hipMalloc(&keep, sizeof(int) * (count + 1)) != hipSuccess ||
+      hipMalloc(&pos, sizeof(int) * (count + 1)) != hipSuccess) {
+    release();
+    set_error(SBLAS_ERR_ALLOC, "sparse-x filter: allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, keep, pos, count + 1, s);
+  if (e == hipSuccess && hipMalloc(&temp, temp_bytes ? temp_bytes : 1) != hipSuccess) {
+    release();
+    set_error(SBLAS_ERR_ALLOC, "sparse-x filter: allocation failed");
+    return SBLAS_ERR_ALLOC;
+  }
+  const dim3 blk(256), grd((count + 256) / 256);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_fill_zero_i32, dim3(1), dim3(1), 0, s, keep + count);
+    if (count > 0) hipLaunchKernelGGL(k_keep_flags, grd, blk, 0, s, count, col, x, keep);
+    e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, keep, pos, count + 1, s);
+  }
+  if (e == hipSuccess) {
+    if (count > 0)
+      hipLaunchKernelGGL(k_scatter_kept, grd, blk, 0, s, count, keep, pos, col, val, col_out,
+                         val_out);
+    hipLaunchKernelGGL(k_rebase_rows, dim3((m + 1 + 255) / 256), blk, 0, s, m, rp, pos, rp_out);
+    e = hipGetLastError();
+  }
+  int total = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&total, pos + count, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  release();
+  if (e != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, std::string("sparse-x filter: ") + hipGetErrorString(e));
+    return SBLAS_ERR_HIP;
+  }
+  *nnz_out = total;
+  return SBLAS_OK;
+}
 
 int coo_sort_device(int m, int n, int64_t nnz, int *row, int *col, void *val, int val_type,
                     bool by_col, hipStream_t s) {

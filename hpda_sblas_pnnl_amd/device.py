@@ -225,3 +225,24 @@ def coo_ptr(sorted_idx, count, stream=None):
     check(lib().sblas_coo_ptr(count, sorted_idx.numel(), sorted_idx.data_ptr(), ptr.data_ptr(),
                               _stream_ptr(stream)), "sblas_coo_ptr")
     return ptr
+
+
+def csr_drop_zero_x(row_ptr, col, val, x, m, n, stream=None):
+    """spMspV's filter on device tensors (src/dspmv_mgpu_v1.cu:19-74): returns
+    (row_ptr, col, val) of the CSR matrix without the elements whose
+    x[col] == 0, rows in order."""
+    torch = _torch()
+    _int32_dev(row_ptr, "row_ptr"), _int32_dev(col, "col")
+    if val.dtype != torch.float64 or x.dtype != torch.float64:
+        raise TypeError("val and x must be float64 device tensors")
+    nnz = col.numel()
+    rp2 = torch.empty(m + 1, dtype=torch.int32, device=col.device)
+    c2 = torch.empty(max(nnz, 1), dtype=torch.int32, device=col.device)
+    v2 = torch.empty(max(nnz, 1), dtype=torch.float64, device=col.device)
+    kept = ctypes.c_int64()
+    check(lib().sblas_csr_drop_zero_x(m, n, nnz, row_ptr.data_ptr(), col.data_ptr(),
+                                      val.data_ptr(), x.data_ptr(), rp2.data_ptr(),
+                                      c2.data_ptr(), v2.data_ptr(), ctypes.byref(kept),
+                                      _stream_ptr(stream)), "sblas_csr_drop_zero_x")
+    k = kept.value
+    return rp2, c2[:k], v2[:k]

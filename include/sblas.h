@@ -327,6 +327,14 @@ int sblas_coo_sort(int m, int n, int64_t nnz, int *d_row_idx, int *d_col_idx, vo
  * column-sorted COO). */
 int sblas_coo_ptr(int count, int64_t nnz, const int *d_sorted_idx, int *d_ptr, void *stream);
 
+/* spMspV's filter (src/dspmv_mgpu_v1.cu:19-74, src/dspmv_mgpu_v2.cu:35-101)
+ * on the device: the CSR matrix without the elements whose d_x[col] == 0,
+ * rows in order.  d_col_out / d_val_out hold at least nnz elements,
+ * d_row_ptr_out m+1; *nnz_out = elements kept.  Returns when done. */
+int sblas_csr_drop_zero_x(int m, int n, int64_t nnz, const int *d_row_ptr, const int *d_col,
+                          const double *d_val, const double *d_x, int *d_row_ptr_out,
+                          int *d_col_out, double *d_val_out, int64_t *nnz_out, void *stream);
+
 /* include/mmio.h:254-367 and test/dspmv_test_dgx1.cu:133-199 -- a Matrix
  * Market coordinate file as COO: real, integer (stored as double) or pattern
  * (1.0) fields, 0-based indices, entries in file order; symmetric,

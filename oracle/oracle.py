@@ -302,6 +302,16 @@ def coo_to_csc(m, n, row, col, val):
     return v, coo_ptr(n, c), r.astype(np.int32)
 
 
+def csr_drop_zero_x(m, rp, col, val, x):
+    """spMspV's filter (src/dspmv_mgpu_v1.cu:19-74): the CSR matrix without the
+    elements whose x[col] == 0 (-0.0 included, NaN kept), rows in order.
+    Returns (row_ptr, col, val)."""
+    rp, col, val, x = np.asarray(rp), np.asarray(col), np.asarray(val), np.asarray(x)
+    keep = x[col] != 0.0
+    kept = np.concatenate([[0], np.cumsum(keep)]).astype(np.int64)
+    return kept[rp].astype(np.int32), col[keep].astype(np.int32), val[keep]
+
+
 def err_normwise(y, r) -> float:
     y, r = _f64(y), _f64(r)
     return lib().ora_err_normwise(len(y), _p(y), _p(r))

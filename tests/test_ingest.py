@@ -150,3 +150,26 @@ def test_helpers_reject_bad_arguments_without_touching_arrays():
         coo2csr(2, -2, 2, val, row, col)
     with pytest.raises(TypeError):
         sortCOORow(2, 2, 2, val.astype(np.float32), row, col)
+
+
+def test_oracle_drop_zero_x_brute_force(oracle):
+    """The filter against the reference's host loop (src/dspmv_mgpu_v1.cu:
+    35-46) restated as plain Python: -0.0 dropped, NaN kept."""
+    rng = np.random.default_rng(6)
+    m, n = 50, 40
+    lens = rng.integers(0, 9, m)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, n, rp[-1]).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1])
+    x = rng.uniform(-1, 1, n)
+    x[rng.random(n) < 0.4] = 0.0
+    x[3], x[4] = -0.0, np.nan
+    rp2, c2, v2 = [0], [], []
+    for i in range(m):
+        for j in range(rp[i], rp[i + 1]):
+            if x[col[j]] != 0.0:
+                v2.append(val[j])
+                c2.append(col[j])
+        rp2.append(len(v2))
+    r, c, v = oracle.csr_drop_zero_x(m, rp, col, val, x)
+    assert r.tolist() == rp2 and c.tolist() == c2 and v.tolist() == v2
