@@ -134,6 +134,7 @@ SIGNATURES = {
     "coo2csc": (None, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "sblas_coo_sort": (_I, [_I, _I, _I64, _P, _P, _P, _I, _I, _P]),
     "sblas_coo_ptr": (_I, [_I, _I64, _P, _P, _P]),
+    "sblas_stream_spmv_csr": (_I, [_I, _I, _I64, _P, _P, _P, _D, _P, _D, _P, _I, _I, _P]),
     "sblas_csr_drop_zero_x": (_I, [_I, _I, _I64, _P, _P, _P, _P, _P, _P, _P,
                                    ctypes.POINTER(_I64), _P]),
     "sblas_mm_read_coo": (_I, [ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I),

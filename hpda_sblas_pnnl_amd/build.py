@@ -33,6 +33,7 @@ HIP_SOURCES = [
     "batch.cpp",
     "ingest.cpp",
     "mmread.cpp",
+    "stream.cpp",
     "plan.cpp",
     "dropin.cpp",
     "partition.cpp",

@@ -181,6 +181,72 @@ spmv_ret run_format(int format, int m, int n, int nnz, const double *alpha, cons
   return ret;
 }
 
+// v2 with the matrix left in host memory: tasks streamed through the device
+// (stream.cpp).  comm_time = x and y transfers, comp_time = the streamed tasks
+// (their copies overlap their kernels), merg_time is inside comp_time.
+spmv_ret run_v2_streamed(int m, int n, int nnz, const double *alpha, const double *val,
+                         const int *rp, const int *col, const double *x, const double *beta,
+                         double *y, int nb, int q) {
+  spmv_ret ret{0, 0, 0, 0, 0};
+  DeviceGuard guard(env_int("SBLAS_DEVICE", 0));
+  if (!guard.ok) {
+    set_error(SBLAS_ERR_NO_DEVICE, "no HIP device");
+    return nan_ret();
+  }
+  OwnedStream os;
+  if (!os.s) {
+    set_error(SBLAS_ERR_HIP, "stream creation failed");
+    return nan_ret();
+  }
+  hipStream_t s = os.s;
+  DeviceArrays mem;
+  double *d_x = mem.alloc<double>(n), *d_y = mem.alloc<double>(m);
+  if (!d_x || !d_y) {
+    set_error(SBLAS_ERR_ALLOC, "device allocation failed");
+    return nan_ret();
+  }
+  EventPair up, down;
+  (void)hipEventRecord(up.a, s);
+  bool ok = n == 0 || hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, s) ==
+                          hipSuccess;
+  ok = ok && (m == 0 || hipMemcpyAsync(d_y, y, sizeof(double) * m, hipMemcpyHostToDevice, s) ==
+                            hipSuccess);
+  (void)hipEventRecord(up.b, s);
+  if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "upload failed");
+    return nan_ret();
+  }
+  const double t0 = now_s();
+  if (stream_spmv_csr(m, n, nnz, rp, col, val, *alpha, d_x, *beta, d_y, nb, q, s, nullptr) !=
+      SBLAS_OK)
+    return nan_ret();
+  ret.comp_time = now_s() - t0;
+  (void)hipEventRecord(down.a, s);
+  if (m > 0 && hipMemcpyAsync(y, d_y, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "download failed");
+    return nan_ret();
+  }
+  (void)hipEventRecord(down.b, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "download failed");
+    return nan_ret();
+  }
+  ret.comm_time = up.seconds() + down.seconds();
+  return ret;
+}
+
+// The v2 drop-in streams its tasks from host memory when SBLAS_V2_STREAM=1,
+// or, unless SBLAS_V2_STREAM=0, when the in-core form would not fit in the
+// device memory that is free.
+bool v2_streams(int m, int n, int64_t nnz) {
+  const char *e = std::getenv("SBLAS_V2_STREAM");
+  if (e && *e) return std::atoi(e) != 0;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+  const double need = 1.1 * (12.0 * nnz + 4.0 * (m + 1.0) + 8.0 * n + 8.0 * m);
+  return need > 0.85 * static_cast<double>(free_b);
+}
+
 spmv_ret run_csr(int m, int n, int nnz, const double *alpha, const double *val, const int *rp,
                  const int *col, const double *x, const double *beta, double *y, int kernel,
                  int part_mode, int nb, int q, double host_prep_s) {
@@ -273,6 +339,9 @@ spmv_ret spMV_mgpu_v2(int m, int n, int nnz, double *alpha, double *csrVal, int 
     set_error(SBLAS_ERR_ARGS, "nb and copy_of_workspace must be >= 1");
     return nan_ret();
   }
+  if (copy_of_workspace <= 64 && v2_streams(m, n, nnz))
+    return run_v2_streamed(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, nb,
+                           copy_of_workspace);
   // tasks go to copy_of_workspace streams when there are few of them;
   // a large task count runs as one fused launch (same results)
   const int T = sblas::v2_num_tasks(nnz, nb, copy_of_workspace);

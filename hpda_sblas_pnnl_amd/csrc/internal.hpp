@@ -79,6 +79,13 @@ int launch_csr_blocks_mrhs(int val_type, int k, const StreamLaunch &L, const Row
                            double *Y, double alpha, double beta, double *partials,
                            hipStream_t s);
 
+// Out-of-core CSR SpMV (stream.cpp): host-resident matrix streamed in the
+// reference's v2 tasks (nb, q) through q device buffers; x, y on the device.
+// Synchronous on s.  *tasks_out (may be null) = task count.
+int stream_spmv_csr(int m, int n, int64_t nnz, const int *h_rp, const int *h_col,
+                    const double *h_val, double alpha, const double *d_x, double beta,
+                    double *d_y, int nb, int q, hipStream_t s, int *tasks_out);
+
 // ---- launchers (sblas_kernels.hip) --------------------------------------------
 struct VectorLaunch {
   int subwave;          // lanes per row: 2..64
@@ -118,6 +125,9 @@ int coo_sort_device(int m, int n, int64_t nnz, int *row, int *col, void *val, in
 int csr_drop_zero_x_device(int m, int n, int64_t nnz, const int *rp, const int *col,
                            const double *val, const double *x, int *rp_out, int *col_out,
                            double *val_out, int64_t *nnz_out, hipStream_t s);
+// Asynchronous on s: column indices outside [0, n) are replaced by 0 and set
+// *bad to 1 (the caller clears it), so a kernel reading x[col] stays in bounds.
+int launch_sanitize_cols(int count, int *col, int n, int *bad, hipStream_t s);
 // ptr[count + 1]: ptr[r] = first position of r in the sorted idx[nnz].
 int ptr_from_sorted_device(int count, int64_t nnz, const int *idx, int *ptr, hipStream_t s);
 

@@ -113,8 +113,6 @@ void *Plan::dev_alloc(size_t bytes) {
   return p;
 }
 
-namespace {
-
 // Vector-kernel launch shape over a set of execution chunks: a block owns
 // rows_per_block consecutive rows of one chunk.
 VectorLaunch vector_shape(const std::vector<ExecChunk> &ex, double mean, int subwave,
@@ -174,6 +172,8 @@ void merge_lists(const std::vector<ExecChunk> &ex, const int *rp, std::vector<in
     if (last != first && split(last)) add(last, static_cast<int>(2 * c + 1));
   }
 }
+
+namespace {
 
 template <typename T>
 int upload(Plan &p, T **dst, const std::vector<T> &src) {

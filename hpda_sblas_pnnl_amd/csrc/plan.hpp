@@ -20,6 +20,14 @@ std::vector<RowBlock> build_row_blocks(const std::vector<ExecChunk> &ex, const i
                                        int cap);
 const char *last_message();
 void clear_error();
+// Vector-kernel (sub-wave) launch shape over execution chunks; dev receives
+// one DevChunk per chunk.
+VectorLaunch vector_shape(const std::vector<ExecChunk> &ex, double mean, int subwave,
+                          int xcd_remap, std::vector<DevChunk> &dev);
+// Rows that more than one chunk touches, with their partial slots in chunk
+// order (slot 2c: chunk c's first row, 2c+1: its last row).
+void merge_lists(const std::vector<ExecChunk> &ex, const int *rp, std::vector<int> &rows,
+                 std::vector<int> &ptr, std::vector<int> &slots);
 
 struct Plan {
   int format = SBLAS_FMT_CSR;

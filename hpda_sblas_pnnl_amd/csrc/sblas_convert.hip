@@ -121,7 +121,26 @@ __global__ void k_rebase_rows(int m, const int *__restrict__ rp, const int *__re
   rp_out[r] = pos[rp[r]];
 }
 
+// column indices outside [0, n) become 0 and raise *bad (never cleared here)
+__global__ void k_sanitize_cols(int count, int *__restrict__ col, int n, int *__restrict__ bad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int c = col[j];
+  if (c < 0 || c >= n) {
+    col[j] = 0;
+    atomicOr(bad, 1);
+  }
+}
+
 }  // namespace
+
+int launch_sanitize_cols(int count, int *col, int n, int *bad, hipStream_t s) {
+  if (count <= 0) return SBLAS_OK;
+  hipLaunchKernelGGL(k_sanitize_cols, dim3((count + 255) / 256), dim3(256), 0, s, count, col, n,
+                     bad);
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
 
 int csr_drop_zero_x_device(int m, int n, int64_t nnz, const int *rp, const int *col,
                            const double *val, const double *x, int *rp_out, int *col_out,

@@ -246,3 +246,21 @@ def csr_drop_zero_x(row_ptr, col, val, x, m, n, stream=None):
                                       _stream_ptr(stream)), "sblas_csr_drop_zero_x")
     k = kept.value
     return rp2, c2[:k], v2[:k]
+
+
+def stream_spmv_csr(m, n, row_ptr, col, val, alpha, x, beta, y, nb, q=2, stream=None):
+    """Out-of-core CSR SpMV in the reference's v2 task structure
+    (src/dspmv_mgpu_v2.cu:103-544): the matrix (numpy arrays in host memory)
+    streams through the device in tasks of nb elements over q streams;
+    x and y are float64 device tensors, y updated in place."""
+    import numpy as np
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col, dtype=np.int32)
+    va = np.ascontiguousarray(val, dtype=np.float64)
+    torch = _torch()
+    if x.dtype != torch.float64 or y.dtype != torch.float64 or not (x.is_cuda and y.is_cuda):
+        raise TypeError("x and y must be float64 device tensors")
+    check(lib().sblas_stream_spmv_csr(m, n, len(va), rp.ctypes.data, ci.ctypes.data,
+                                      va.ctypes.data, float(alpha), x.data_ptr(), float(beta),
+                                      y.data_ptr(), int(nb), int(q), _stream_ptr(stream)),
+          "sblas_stream_spmv_csr")

@@ -327,6 +327,20 @@ int sblas_coo_sort(int m, int n, int64_t nnz, int *d_row_idx, int *d_col_idx, vo
  * column-sorted COO). */
 int sblas_coo_ptr(int count, int64_t nnz, const int *d_sorted_idx, int *d_ptr, void *stream);
 
+/* src/dspmv_mgpu_v2.cu:103-544 task structure, out of core: the CSR matrix
+ * stays in host memory (h_*) and streams through the device in the v2 tasks
+ * of nb elements (nb clamped to 8e8/q as in the reference) over q <= 64
+ * streams; the device holds the row pointers, x, y (d_x, d_y) and q task
+ * buffers, so the matrix may exceed device memory.  Kernel 1 on every task;
+ * rows cut by task edges are merged on the device in task order.  Returns
+ * when done.  An out-of-range column index gives SBLAS_ERR_ARGS (y is then
+ * unspecified, the read of x stays in bounds).  spMV_mgpu_v2 takes this path
+ * when SBLAS_V2_STREAM=1, or when the matrix does not fit the free device
+ * memory (unless SBLAS_V2_STREAM=0). */
+int sblas_stream_spmv_csr(int m, int n, int64_t nnz, const int *h_row_ptr, const int *h_col,
+                          const double *h_val, double alpha, const double *d_x, double beta,
+                          double *d_y, int nb, int q, void *stream);
+
 /* spMspV's filter (src/dspmv_mgpu_v1.cu:19-74, src/dspmv_mgpu_v2.cu:35-101)
  * on the device: the CSR matrix without the elements whose d_x[col] == 0,
  * rows in order.  d_col_out / d_val_out hold at least nnz elements,
