@@ -356,11 +356,31 @@ def extras_dropin(A, x, y0, alpha, beta, log_to):
         r = S.spMV_mgpu_v1_numa(A.m, A.n, A.nnz, alpha, A.val, A.row_ptr, A.col, x, beta, y,
                                 ngpu=1, kernel=1)
         wall = time.perf_counter() - t0
-        out = {"wall_ms": round(wall * 1e3, 2),
+        out = {"wall_ms": round(wall * 1e3, 2), "entry": "spMV_mgpu_v1_numa",
                "GFLOPs_pcie_inclusive": round(2 * A.nnz / wall / 1e9, 2),
                "part_ms": round(r.part_time * 1e3, 3), "comm_ms": round(r.comm_time * 1e3, 3),
                "comp_ms": round(r.comp_time * 1e3, 3), "merg_ms": round(r.merg_time * 1e3, 3)}
     log_to(f"[bench] drop-in spMV_mgpu_v1_numa: {out}")
+    # v2 with the matrix streamed from host memory in nine tasks over four
+    # streams (copies of one task overlap kernels of the others), beside the
+    # in-core v2 drop-in at the same tasks
+    nb = (A.nnz + 8) // 9
+    for mode, key in (("1", "v2_streamed"), ("0", "v2_in_core")):
+        os.environ["SBLAS_V2_STREAM"] = mode
+        try:
+            for rep in range(2):
+                y = y0.copy()
+                t0 = time.perf_counter()
+                r = S.spMV_mgpu_v2(A.m, A.n, A.nnz, alpha, A.val, A.row_ptr, A.col, x, beta, y,
+                                   ngpu=1, kernel=1, nb=nb, copy_of_workspace=4)
+                wall = time.perf_counter() - t0
+        finally:
+            del os.environ["SBLAS_V2_STREAM"]
+        out[key] = {"wall_ms": round(wall * 1e3, 2), "tasks": 9, "streams": 4,
+                    "GFLOPs_pcie_inclusive": round(2 * A.nnz / wall / 1e9, 2),
+                    "comm_ms": round(r.comm_time * 1e3, 3), "comp_ms": round(r.comp_time * 1e3, 3),
+                    "host_to_device_GBs": round((A.nnz * 12 + (A.m + 1) * 4) / wall / 1e9, 1)}
+        log_to(f"[bench] drop-in spMV_mgpu_v2 {key}: {out[key]}")
     return out
 
 
