@@ -361,9 +361,16 @@ def extras_dropin(A, x, y0, alpha, beta, log_to):
                "part_ms": round(r.part_time * 1e3, 3), "comm_ms": round(r.comm_time * 1e3, 3),
                "comp_ms": round(r.comp_time * 1e3, 3), "merg_ms": round(r.merg_time * 1e3, 3)}
     log_to(f"[bench] drop-in spMV_mgpu_v1_numa: {out}")
-    # v2 with the matrix streamed from host memory in nine tasks over four
-    # streams (copies of one task overlap kernels of the others), beside the
-    # in-core v2 drop-in at the same tasks
+    out.update(extras_v2_stream(A, x, y0, alpha, beta, log_to))
+    return out
+
+
+def extras_v2_stream(A, x, y0, alpha, beta, log_to):
+    """spMV_mgpu_v2 with the matrix streamed from host memory in nine tasks
+    over four streams (copies of one task overlap kernels of the others),
+    beside the in-core v2 drop-in at the same tasks; PCIe-inclusive."""
+    from hpda_sblas_pnnl_amd import spmv as S
+    out = {}
     nb = (A.nnz + 8) // 9
     for mode, key in (("1", "v2_streamed"), ("0", "v2_in_core")):
         os.environ["SBLAS_V2_STREAM"] = mode
@@ -409,6 +416,10 @@ def main():
                     help="skip the R-MAT precision and batched measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device copy first (reference for HBM byte counters)")
+    ap.add_argument("--only", choices=["ingest", "stream"],
+                    help="run one side measurement alone and print its JSON (for profiling): "
+                         "device COO ingestion of R-MAT scale 22, or the streamed v2 drop-in "
+                         "on the banded matrix")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -421,6 +432,16 @@ def main():
 
     assert torch.cuda.is_available(), "no HIP device"
     dev = torch.device("cuda:0")
+    if args.only == "ingest":
+        from hpda_sblas_pnnl_amd import gen as G
+        A = G.rmat_csr(scale=22, edge_factor=17, seed=0)
+        rp, col, val = (torch.from_numpy(v).to(dev) for v in (A.row_ptr, A.col, A.val))
+        print(json.dumps({"ingest": extras_ingest(A, rp, col, val, log)}), flush=True)
+        return
+    if args.only == "stream":
+        A, x, y0, alpha, beta, _, _, _ = make_workload("banded")
+        print(json.dumps({"stream": extras_v2_stream(A, x, y0, alpha, beta, log)}), flush=True)
+        return
     if args.copy_probe:
         # a plain copy of known size: reads 1 GiB, writes 1 GiB
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
