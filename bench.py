@@ -30,6 +30,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU leg's OpenMP threads one per core, packed (read when the runtime loads)
+os.environ.setdefault("OMP_PLACES", "cores")
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 PEAK_HBM_GBS = 8000.0
 
@@ -102,7 +105,32 @@ def cpu_baseline(A, x, y0, alpha, beta, budget_s=20.0):
                    f"({cores} threads, {t_all * 1e3:.1f} ms each); one serial pass "
                    f"{t_serial * 1e3:.1f} ms = {2.0 * A.nnz / t_serial / 1e9:.3f} GFLOP/s"),
         "serial_gflops": round(2.0 * A.nnz / t_serial / 1e9, 3),
+        "host": host_description(),
     }, y
+
+
+def host_description():
+    """The host the CPU leg ran on: CPU model, logical CPUs of the machine and of
+    this process, NUMA nodes, and the OpenMP binding variables in effect."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and
+                    d[4:].isdigit()])
+    except OSError:
+        numa = None
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(),
+            "process_cpus": len(os.sched_getaffinity(0)), "numa_nodes": numa,
+            "OMP_PLACES": os.environ.get("OMP_PLACES"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _time_calls(fn, reps=20, warmup=3):
