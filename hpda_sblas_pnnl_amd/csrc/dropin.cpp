@@ -138,6 +138,10 @@ spmv_ret run_format(int format, int m, int n, int nnz, const double *alpha, cons
   o.nb = nb;
   o.streams = q;
   o.chunks = std::max(1, env_int("SBLAS_CHUNKS", 1));
+  // one SpMV per call: kernel 1 runs in the row-block form, which needs no
+  // format build (the column-panel form's build costs far more than its
+  // kernel saves in a single product); SBLAS_VARIANT picks another form
+  o.variant = env_int("SBLAS_VARIANT", 2);
   sblas_plan plan = nullptr;
   const double t_plan0 = now_s();
   int st = format == SBLAS_FMT_CSR
