@@ -30,9 +30,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# the CPU leg's OpenMP threads one per core, packed (read when the runtime loads)
-os.environ.setdefault("OMP_PLACES", "cores")
-os.environ.setdefault("OMP_PROC_BIND", "close")
 
 PEAK_HBM_GBS = 8000.0
 
