@@ -34,6 +34,7 @@ HIP_SOURCES = [
     "ingest.cpp",
     "mmread.cpp",
     "stream.cpp",
+    "staging.cpp",
     "plan.cpp",
     "dropin.cpp",
     "partition.cpp",

@@ -53,4 +53,22 @@ struct OwnedStream {
   OwnedStream &operator=(const OwnedStream &) = delete;
 };
 
+// Host-to-device copies of pageable host arrays through a ring of pinned
+// buffers (staging.cpp): host threads fill one slot while earlier slots'
+// transfers run.  to_device issues the transfers on s and returns once the
+// last piece is queued; the ring waits for every transfer before it frees
+// its buffers.
+struct PinnedRing {
+  std::vector<void *> slots;
+  std::vector<hipEvent_t> events;
+  std::vector<bool> used;
+  size_t next = 0;
+  PinnedRing() = default;
+  PinnedRing(const PinnedRing &) = delete;
+  PinnedRing &operator=(const PinnedRing &) = delete;
+  ~PinnedRing();
+  int init();
+  int to_device(void *dst, const void *src, size_t bytes, hipStream_t s);
+};
+
 }  // namespace sblas

@@ -1,0 +1,92 @@
+// staging.cpp -- host-to-device copies of the caller's (pageable) arrays
+// through a ring of pinned buffers.
+//
+// A copy from pageable memory is staged by the runtime at about half the PCIe
+// rate.  Here each copy is cut into pieces; host threads copy a piece into a
+// free pinned slot while earlier slots' DMA transfers run, and a slot is
+// refilled only after the event recorded behind its last transfer has
+// completed.  Transfers go to the stream of the copy, so the kernels that
+// consume them can start while later pieces are still on their way.
+#include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "host_util.hpp"
+#include "internal.hpp"
+
+namespace sblas {
+namespace {
+
+constexpr size_t kSlotBytes = size_t{64} << 20;  // one piece
+constexpr int kSlots = 3;
+constexpr size_t kThreadMin = size_t{4} << 20;   // below this one thread copies
+
+void parallel_copy(void *dst, const void *src, size_t bytes) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t T = std::min<size_t>({8, hw, bytes / kThreadMin});
+  if (T <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (bytes + T - 1) / T;
+  for (size_t i = 0; i < T; i++) {
+    const size_t lo = i * per, hi = std::min(bytes, lo + per);
+    if (lo >= hi) break;
+    th.emplace_back([=] {
+      std::memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, hi - lo);
+    });
+  }
+  for (std::thread &t : th) t.join();
+}
+
+}  // namespace
+
+PinnedRing::~PinnedRing() {
+  for (size_t k = 0; k < slots.size(); k++) {
+    if (used[k]) (void)hipEventSynchronize(events[k]);
+    (void)hipHostFree(slots[k]);
+    (void)hipEventDestroy(events[k]);
+  }
+}
+
+int PinnedRing::init() {
+  if (!slots.empty()) return SBLAS_OK;
+  for (int k = 0; k < kSlots; k++) {
+    void *p = nullptr;
+    hipEvent_t e = nullptr;
+    if (hipHostMalloc(&p, kSlotBytes, hipHostMallocDefault) != hipSuccess) {
+      set_error(SBLAS_ERR_ALLOC, "pinned staging: allocation failed");
+      return SBLAS_ERR_ALLOC;
+    }
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      (void)hipHostFree(p);
+      set_error(SBLAS_ERR_HIP, "pinned staging: event creation failed");
+      return SBLAS_ERR_HIP;
+    }
+    slots.push_back(p);
+    events.push_back(e);
+    used.push_back(false);
+  }
+  return SBLAS_OK;
+}
+
+int PinnedRing::to_device(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return SBLAS_OK;
+  int st = init();
+  if (st != SBLAS_OK) return st;
+  for (size_t off = 0; off < bytes; off += kSlotBytes) {
+    const size_t len = std::min(kSlotBytes, bytes - off);
+    const size_t k = next++ % slots.size();
+    if (used[k]) SBLAS_HIP_TRY(hipEventSynchronize(events[k]));  // its last transfer is done
+    parallel_copy(slots[k], static_cast<const char *>(src) + off, len);
+    SBLAS_HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + off, slots[k], len,
+                                 hipMemcpyHostToDevice, s));
+    SBLAS_HIP_TRY(hipEventRecord(events[k], s));
+    used[k] = true;
+  }
+  return SBLAS_OK;
+}
+
+}  // namespace sblas

@@ -8,9 +8,10 @@
 // stays in the caller's host memory; the device holds the row pointers, x, y
 // and q task buffers, so a matrix larger than device memory runs.  Task t goes
 // to stream t mod q: its columns and values are copied into that stream's
-// buffer, out-of-range columns are replaced by 0 and flagged (the kernel then
-// reads x in bounds; the call reports SBLAS_ERR_ARGS), and the sub-wave CSR
-// kernel runs the task's rows over the task's element window.  Copies on one
+// buffer (through pinned staging), out-of-range columns are replaced by 0 and
+// flagged (the kernel then reads x in bounds; the call reports
+// SBLAS_ERR_ARGS), and the sub-wave CSR kernel runs the task's rows over the
+// task's element window.  Copies on one
 // stream overlap kernels on the others.  Rows cut by task edges leave
 // partials that one merge adds in task order after every stream has
 // finished, so the results equal those of the in-core plan at the same tasks.
@@ -126,6 +127,7 @@ int stream_spmv_csr(int m, int n, int64_t nnz, const int *h_rp, const int *h_col
     S.st.push_back(x);
     S.ev.push_back(e);
   }
+  PinnedRing ring;  // destroyed (after its transfers) before the streams
   hipEvent_t start = nullptr;
   if (hipEventCreateWithFlags(&start, hipEventDisableTiming) != hipSuccess) {
     set_error(SBLAS_ERR_HIP, "stream spmv: event creation failed");
@@ -139,12 +141,12 @@ int stream_spmv_csr(int m, int n, int64_t nnz, const int *h_rp, const int *h_col
     const int k = t % nq;
     hipStream_t ts = S.st[k];
     const int64_t lo = ex[t].lo, len = ex[t].hi - ex[t].lo;
-    // a buffer is reused by task t + nq on the same stream, after task t's kernel
-    SBLAS_HIP_TRY(hipMemcpyAsync(d_col[k], h_col + lo, sizeof(int) * len, hipMemcpyHostToDevice,
-                                 ts));
-    SBLAS_HIP_TRY(hipMemcpyAsync(d_val[k], h_val + lo, sizeof(double) * len,
-                                 hipMemcpyHostToDevice, ts));
-    int st = launch_sanitize_cols(static_cast<int>(len), d_col[k], n, d_bad, ts);
+    // a buffer is reused by task t + nq on the same stream, after task t's
+    // kernel; the copies go through pinned staging (host threads fill the
+    // next slots while this task's transfers and the other streams' kernels run)
+    int st = ring.to_device(d_col[k], h_col + lo, sizeof(int) * len, ts);
+    if (st == SBLAS_OK) st = ring.to_device(d_val[k], h_val + lo, sizeof(double) * len, ts);
+    if (st == SBLAS_OK) st = launch_sanitize_cols(static_cast<int>(len), d_col[k], n, d_bad, ts);
     // the kernel addresses elements by their global index: the buffers are
     // passed rebased by the task's first element
     if (st == SBLAS_OK)
