@@ -60,16 +60,22 @@ struct sblas_batch_s {
   int *d_qbound = nullptr;
   int *d_gmat = nullptr, *d_grow = nullptr, *d_gptr = nullptr, *d_gslot = nullptr;
   int groups = 0;
-  int grid = 0;
+  int n_panel = 0;                 // items [0, n_panel) are row panels, the rest row blocks
+  int grid_panel = 0, grid_rows = 0;
   int items_per_thread = 8;
   std::vector<void *> bufs;
   hipStream_t cap_stream = nullptr;
+  hipStream_t side = nullptr;          // the row-block launch runs beside the panels
+  hipEvent_t fork = nullptr, join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   ~sblas_batch_s() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    if (side) (void)hipStreamDestroy(side);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
     for (void *p : bufs) (void)hipFree(p);
     for (sblas::PanelFormat &f : panels)
       for (void *p : f.buffers) (void)hipFree(p);
@@ -92,9 +98,28 @@ struct sblas_batch_s {
     return SBLAS_OK;
   }
   int enqueue(hipStream_t s) {
-    int st = sblas::launch_batch(items_per_thread, grid, static_cast<int>(items.size()), d_items,
-                                 d_mats, d_partials, d_qbound, d_ctr, s);
+    // row panels and row blocks: each kind in its own persistent launch with
+    // its own queues (qbound) and counters (ctr); with both kinds present the
+    // row blocks run on a side stream forked from s and joined back before the
+    // merge, so the two launches share the GPU instead of queueing
+    constexpr int Q = sblas::kBatchQueues + 1;
+    const int n_rows = static_cast<int>(items.size()) - n_panel;
+    const bool both = grid_panel > 0 && grid_rows > 0;
+    hipStream_t rs = both ? side : s;
+    if (both) {
+      SBLAS_HIP_TRY(hipEventRecord(fork, s));
+      SBLAS_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+    }
+    int st = sblas::launch_batch(sblas::kBatchPanel, items_per_thread, grid_panel, n_panel,
+                                 d_items, d_mats, d_partials, d_qbound, d_ctr, s);
     if (st != SBLAS_OK) return st;
+    st = sblas::launch_batch(sblas::kBatchRowBlock, items_per_thread, grid_rows, n_rows,
+                             d_items + n_panel, d_mats, d_partials, d_qbound + Q, d_ctr + Q, rs);
+    if (st != SBLAS_OK) return st;
+    if (both) {
+      SBLAS_HIP_TRY(hipEventRecord(join, side));
+      SBLAS_HIP_TRY(hipStreamWaitEvent(s, join, 0));
+    }
     return sblas::launch_batch_merge(groups, d_gmat, d_grow, d_gptr, d_gslot, d_mats, d_partials,
                                      s);
   }
@@ -247,14 +272,17 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
                                             rs.cv, rs.max_row});
     h_rps[i] = std::move(h_rp);
   }
-  // list order: matrix by matrix, each matrix's items in its own order (row
-  // panels of one matrix stay neighbours, so they share x in one XCD's L2)
+  // list order: the row-panel items, then the row-block items (one persistent
+  // launch each); within a kind matrix by matrix, each matrix's items in its
+  // own order (row panels of one matrix stay neighbours, so they share x in
+  // one XCD's L2)
   std::vector<std::pair<int, int>> order;  // (matrix, index in its list)
-  for (int i = 0; i < count; i++)
-    for (int q = 0; q < static_cast<int>(pending[i].size()); q++) order.emplace_back(i, q);
+  for (int kind : {kBatchPanel, kBatchRowBlock})
+    for (int i = 0; i < count; i++)
+      for (int q = 0; q < static_cast<int>(pending[i].size()); q++)
+        if (pending[i][q].item.kind == kind) order.emplace_back(i, q);
   std::vector<std::vector<int>> where(count);
   for (int i = 0; i < count; i++) where[i].assign(pending[i].size(), -1);
-  int64_t total_work = 0;
   int64_t nslots = 0;
   for (const auto &o : order) {
     where[o.first][o.second] = static_cast<int>(b->items.size());
@@ -263,22 +291,34 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     item.pslot = static_cast<int>(nslots);  // 2 slots of kv partials each
     nslots += 2 * kv;
     b->items.push_back(item);
-    total_work += pending[o.first][o.second].work * kv + 256;  // elements + per-item cost
   }
   if (nslots > INT32_MAX) {
     set_error(SBLAS_ERR_ARGS, "batch: too many work items");
     return fail(SBLAS_ERR_ARGS);
   }
-  // eight queues of about equal work, contiguous in the list
-  std::vector<int> qbound(kBatchQueues + 1, static_cast<int>(b->items.size()));
-  qbound[0] = 0;
-  {
+  // per kind, eight queues of about equal work, contiguous in that kind's
+  // part of the list (bounds relative to its first item)
+  size_t n_panel = 0;
+  while (n_panel < b->items.size() && b->items[n_panel].kind == kBatchPanel) n_panel++;
+  b->n_panel = static_cast<int>(n_panel);
+  std::vector<int> qbound;
+  for (int part = 0; part < 2; part++) {
+    const size_t w0 = part == 0 ? 0 : n_panel, w1 = part == 0 ? n_panel : order.size();
+    auto work = [&](size_t w) {  // elements + per-item cost
+      return pending[order[w].first][order[w].second].work * b->mats[order[w].first].k + 256;
+    };
+    int64_t total_work = 0;
+    for (size_t w = w0; w < w1; w++) total_work += work(w);
+    std::vector<int> qb(kBatchQueues + 1, static_cast<int>(w1 - w0));
+    qb[0] = 0;
     int64_t run = 0;
     int q = 1;
-    for (size_t w = 0; w < order.size() && q < kBatchQueues; w++) {
-      while (q < kBatchQueues && run >= total_work * q / kBatchQueues) qbound[q++] = static_cast<int>(w);
-      run += pending[order[w].first][order[w].second].work * b->mats[order[w].first].k + 256;
+    for (size_t w = w0; w < w1 && q < kBatchQueues; w++) {
+      while (q < kBatchQueues && run >= total_work * q / kBatchQueues)
+        qb[q++] = static_cast<int>(w - w0);
+      run += work(w);
     }
+    qbound.insert(qbound.end(), qb.begin(), qb.end());
   }
   // rows cut by an item's element window: partial slots pslot (first row) and
   // pslot + k (last row), listed per row in the matrix's own item order
@@ -317,15 +357,24 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     return fail(st);
   std::vector<double> zeros(static_cast<size_t>(std::max<int64_t>(nslots, 1)), 0.0);
   if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);
-  if ((st = b->upload(&b->d_ctr, std::vector<int>(kBatchQueues + 1, 0))) != SBLAS_OK ||
+  if ((st = b->upload(&b->d_ctr, std::vector<int>(2 * (kBatchQueues + 1), 0))) != SBLAS_OK ||
       (st = b->upload(&b->d_qbound, qbound)) != SBLAS_OK)
     return fail(st);
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // the resident workgroups, never more than there are items
-  const int per_cu = batch_occupancy(b->items_per_thread);
-  b->grid = std::min<int>(static_cast<int>(b->items.size()), std::max(1, cus) * per_cu);
+  // per kind the resident workgroups, never more than there are items
+  const int n_rows = static_cast<int>(b->items.size()) - b->n_panel;
+  b->grid_panel = std::min<int>(b->n_panel, std::max(1, cus) *
+                                                batch_occupancy(kBatchPanel, b->items_per_thread));
+  b->grid_rows = std::min<int>(n_rows, std::max(1, cus) *
+                                           batch_occupancy(kBatchRowBlock, b->items_per_thread));
+  if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+    set_error(SBLAS_ERR_HIP, "batch: stream or event creation failed");
+    return fail(SBLAS_ERR_HIP);
+  }
   *out = b;
   return SBLAS_OK;
 }
@@ -371,8 +420,9 @@ int sblas_batch_get_choice(sblas_batch b, int index, sblas_batch_choice *out) {
 int sblas_batch_get_shape(sblas_batch b, int *work_items, int *grid, int *launches) {
   if (!b) return SBLAS_ERR_ARGS;
   if (work_items) *work_items = static_cast<int>(b->items.size());
-  if (grid) *grid = b->grid;
-  if (launches) *launches = (b->items.empty() ? 0 : 1) + (b->groups > 0 ? 1 : 0);
+  if (grid) *grid = std::max(b->grid_panel, b->grid_rows);
+  if (launches)
+    *launches = (b->grid_panel > 0 ? 1 : 0) + (b->grid_rows > 0 ? 1 : 0) + (b->groups > 0 ? 1 : 0);
   return SBLAS_OK;
 }
 

@@ -222,18 +222,18 @@ struct BatchItem {  // one row block, or one row panel, of one matrix
   int pslot;               // first partial slot: 2*k slots (first row, last row)
   int pad_;
 };
-// Persistent launch over all items: the list is cut into kBatchQueues
-// contiguous queues [qbound[q], qbound[q+1]); workgroups take items from
-// per-queue counters ctr[q] (ctr[kBatchQueues] counts finished workgroups;
-// the last one resets them all).  Each item's arithmetic does not depend on
-// which workgroup runs it.
+// Persistent launch over the items of one kind (kBatchPanel or
+// kBatchRowBlock): the list is cut into kBatchQueues contiguous queues
+// [qbound[q], qbound[q+1]); workgroups take items from per-queue counters
+// ctr[q] (ctr[kBatchQueues] counts finished workgroups; the last one resets
+// them all).  Each item's arithmetic does not depend on which workgroup runs it.
 constexpr int kBatchQueues = 8;  // one per XCD
-constexpr int kBatchPanelRowsMax = 1024;  // rows per row panel in a batch (4 per thread)
-int launch_batch(int items_per_thread, int grid, int nitems, const BatchItem *items,
+constexpr int kBatchPanelRowsMax = 1536;  // rows per row panel in a batch (6 per thread)
+int launch_batch(int kind, int items_per_thread, int grid, int nitems, const BatchItem *items,
                  const BatchMatrix *mats, double *partials, const int *qbound, int *ctr,
                  hipStream_t s);
-// Resident workgroups per CU of the batched kernel.
-int batch_occupancy(int items_per_thread);
+// Resident workgroups per CU of the batched kernel for one item kind.
+int batch_occupancy(int kind, int items_per_thread);
 int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,
                        const int *gslot, const BatchMatrix *mats, const double *partials,
                        hipStream_t s);

@@ -237,15 +237,17 @@ __device__ __forceinline__ void rows_typed(int w, const BatchItem &it, const Bat
   }
 }
 
-template <int ITEMS, int BLOCK>
+// One persistent kernel per item kind (MODE kBatchPanel: row panels, else row
+// blocks), so each is compiled with only its own body: its registers and LDS
+// (48 KB for panels, 20-36 KB for row blocks) set its own occupancy.
+template <int MODE, int ITEMS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__restrict__ items,
                                                  const BatchMatrix *__restrict__ mats,
                                                  double *__restrict__ partials,
                                                  const int *__restrict__ qbound,
                                                  int *__restrict__ ctr) {
   static_assert(BLOCK == 256, "the panel body assumes 256 threads");
-  constexpr int LDS_RB = BLOCK * ITEMS + kStreamRowsMax;
-  constexpr int LDS = LDS_RB > panel::kLdsDoubles ? LDS_RB : panel::kLdsDoubles;
+  constexpr int LDS = MODE == kBatchPanel ? panel::kLdsDoubles : BLOCK * ITEMS + kStreamRowsMax;
   __shared__ double lds[LDS];
   __shared__ double wsum[BLOCK / 64];
   __shared__ int next;
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
       if (t == 0) claimed = qbound[q] + atomicAdd(&ctr[q], 1);
       const BatchItem it = items[w];
       const BatchMatrix &M = mats[it.mat];
-      if (it.kind == kBatchPanel) {
+      if constexpr (MODE == kBatchPanel) {
         row_panel_any(it, M, partials, lds);
       } else if (M.val_type == SBLAS_VAL_F64) {  // value type chosen once per item
         rows_typed<double, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
@@ -312,25 +314,32 @@ __global__ void k_batch_merge(int groups, const int *__restrict__ gmat,
 
 }  // namespace
 
-int launch_batch(int items_per_thread, int grid, int nitems, const BatchItem *items,
+int launch_batch(int kind, int items_per_thread, int grid, int nitems, const BatchItem *items,
                  const BatchMatrix *mats, double *partials, const int *qbound, int *ctr,
                  hipStream_t s) {
   if (nitems <= 0 || grid <= 0) return SBLAS_OK;
   constexpr int B = kStreamBlock;
-  if (items_per_thread == 16)
-    hipLaunchKernelGGL((k_batch<16, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats, partials,
-                       qbound, ctr);
+  constexpr int P = kBatchPanel, R = kBatchRowBlock;
+  if (kind == kBatchPanel)
+    hipLaunchKernelGGL((k_batch<P, 8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats,
+                       partials, qbound, ctr);
+  else if (items_per_thread == 16)
+    hipLaunchKernelGGL((k_batch<R, 16, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats,
+                       partials, qbound, ctr);
   else
-    hipLaunchKernelGGL((k_batch<8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats, partials,
-                       qbound, ctr);
+    hipLaunchKernelGGL((k_batch<R, 8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats,
+                       partials, qbound, ctr);
   SBLAS_HIP_TRY(hipGetLastError());
   return SBLAS_OK;
 }
 
-int batch_occupancy(int items_per_thread) {
+int batch_occupancy(int kind, int items_per_thread) {
   int per_cu = 0;
-  const void *fn = items_per_thread == 16 ? reinterpret_cast<const void *>(&k_batch<16, kStreamBlock>)
-                                          : reinterpret_cast<const void *>(&k_batch<8, kStreamBlock>);
+  constexpr int P = kBatchPanel, R = kBatchRowBlock;
+  const void *fn =
+      kind == kBatchPanel      ? reinterpret_cast<const void *>(&k_batch<P, 8, kStreamBlock>)
+      : items_per_thread == 16 ? reinterpret_cast<const void *>(&k_batch<R, 16, kStreamBlock>)
+                               : reinterpret_cast<const void *>(&k_batch<R, 8, kStreamBlock>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kStreamBlock, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;

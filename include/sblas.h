@@ -195,7 +195,7 @@ typedef struct sblas_opts {
   int variant;     /* kernel 1 form: 0 auto, 1 sub-wave per row, 2 row blocks */
                    /* staged in LDS, 3 column panels with x staged in LDS    */
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
-  int panel_rows;  /* column-panel form: rows per row panel (1..2048; 0 auto) */
+  int panel_rows;  /* column-panel form: rows per row panel (1..2048, 1..1536 in a batch; 0 auto) */
   int reserved[5];
 } sblas_opts;
 
@@ -251,11 +251,14 @@ int sblas_plan_set_timing(sblas_plan plan, int on);
 int sblas_plan_kernel_times(sblas_plan plan, float *ms, int cap);
 
 /* ---------------------------------------------------------------------- */
-/* batched matrices: one persistent launch, replayable as a hipGraph        */
+/* batched matrices: persistent launches, replayable as one hipGraph       */
 /* ---------------------------------------------------------------------- */
 /* No counterpart in the reference (its calls take one matrix).  Each item is
  * a CSR matrix on the device with its own x, y, alpha, beta; arrays must stay
- * alive and unchanged while the batch exists. */
+ * alive and unchanged while the batch exists.  A run is one persistent launch
+ * over the matrices' row panels and one over their row blocks (side by side,
+ * the second on a stream forked from the caller's and joined back), then one
+ * merge launch.  A batch runs one call at a time. */
 typedef struct sblas_batch_item {
   int m, n;
   int64_t nnz;
@@ -290,7 +293,8 @@ int sblas_batch_run(sblas_batch batch, void *stream);
 /* The same work as one hipGraph (captured on first use, replayed after). */
 int sblas_batch_graph_launch(sblas_batch batch, void *stream);
 int sblas_batch_get_choice(sblas_batch batch, int index, sblas_batch_choice *out);
-/* work items, persistent grid size, kernel launches per run */
+/* work items, the larger persistent grid of the two launches, kernel
+ * launches per run */
 int sblas_batch_get_shape(sblas_batch batch, int *work_items, int *grid, int *launches);
 int sblas_batch_destroy(sblas_batch batch);
 
