@@ -195,7 +195,8 @@ typedef struct sblas_opts {
   int variant;     /* kernel 1 form: 0 auto, 1 sub-wave per row, 2 row blocks */
                    /* staged in LDS, 3 column panels with x staged in LDS    */
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
-  int panel_rows;  /* column-panel form: rows per row panel (1..2048, 1..1536 in a batch; 0 auto) */
+  int panel_rows;  /* column-panel form: rows per row panel (1..2048, or   */
+                   /* 1..1536 in a batch; 0 auto)                           */
   int reserved[5];
 } sblas_opts;
 
