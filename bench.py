@@ -106,6 +106,12 @@ def cpu_baseline(A, x, y0, alpha, beta, budget_s=20.0):
     }, y
 
 
+def csr5_model_bytes(m, nnz):
+    """(m + 1 + nnz) int32 indices and (2 nnz + m) doubles: values, x once per
+    element, y (include/detail/utils.h:10-14)."""
+    return (m + 1 + nnz) * 4 + (2 * nnz + m) * 8
+
+
 def host_description():
     """The host the CPU leg ran on: CPU model, logical CPUs of the machine and of
     this process, NUMA nodes, and the OpenMP binding variables in effect."""
@@ -537,6 +543,10 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": load_traffic(args.workload),
                      "algorithmic_bytes": cb, "kernel_ms": round(k_ms, 4)},
+        # the CSR5 library's own byte model (include/detail/utils.h:10-14), which
+        # reads x once per element: an upper-bound view, not the roofline
+        "csr5_model": {"bytes": csr5_model_bytes(A.m, A.nnz),
+                       "GBs": round(csr5_model_bytes(A.m, A.nnz) / (k_ms * 1e-3) / 1e9, 1)},
     }
     if not args.no_cpu:
         cpu, y_ref = cpu_baseline(A, x, y0, alpha, beta)
