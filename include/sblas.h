@@ -50,7 +50,10 @@ const char *sblas_version(void);
 /* 1. drop-in entry points                                                 */
 /* ---------------------------------------------------------------------- */
 
-/* include/spmv_kernel.h:16-57: five timings in seconds, same field order. */
+/* include/spmv_kernel.h:16-57: five timings in seconds, same field order.
+ * The C++ compat header spmv_kernel.h defines the same struct with the
+ * reference's helper methods (it sets SBLAS_SPMV_RET_DEFINED first). */
+#ifndef SBLAS_SPMV_RET_DEFINED
 typedef struct spmv_ret {
   double numa_part_time;
   double part_time;
@@ -58,6 +61,7 @@ typedef struct spmv_ret {
   double comm_time;
   double merg_time;
 } spmv_ret;
+#endif
 
 /* src/dspmv_mgpu_baseline.cu:14 -- row blocks */
 spmv_ret spMV_mgpu_baseline(int m, int n, int nnz, double *alpha, double *csrVal,
