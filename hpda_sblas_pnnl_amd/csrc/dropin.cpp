@@ -13,6 +13,7 @@
 // the reference leaves unwritten or scales twice (empty rows at chunk edges,
 // CSC/COO merges that add beta*y per chunk); beta == 0 does not read y.
 #include <chrono>
+#include <cstdio>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
@@ -31,7 +32,17 @@ double now_s() {
       .count();
 }
 
+// The failure return of every entry point: NaN timings, the status left in
+// sblas_last_error().  With SBLAS_EXIT_ON_ERROR=1 the process exits instead,
+// as the reference's checkCudaErrors does (include/detail/cuda/helper_cuda.h:
+// 583-595), after printing the error.
 spmv_ret nan_ret() {
+  const char *e = std::getenv("SBLAS_EXIT_ON_ERROR");
+  if (e && std::atoi(e) != 0) {
+    std::fprintf(stderr, "sblas error %d (%s): %s\n", last_error(),
+                 sblas_error_string(last_error()), last_message());
+    std::exit(EXIT_FAILURE);
+  }
   const double q = std::numeric_limits<double>::quiet_NaN();
   return spmv_ret{q, q, q, q, q};
 }

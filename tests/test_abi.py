@@ -86,3 +86,27 @@ def test_no_device_fails_loudly():
     with pytest.raises(_lib.SblasError) as e:
         spmv.spMV_mgpu_v1(2, 2, 3, 1.0, val, rp, col, x, 0.0, y)
     assert e.value.status in (_lib.SBLAS_ERR_NO_DEVICE, _lib.SBLAS_ERR_HIP)
+
+
+def test_exit_on_error_emulation():
+    """SBLAS_EXIT_ON_ERROR=1 makes a failing entry point exit the process with
+    EXIT_FAILURE after printing the error, as the reference's checkCudaErrors
+    does; unset, the same call returns NaN timings."""
+    import subprocess
+    import sys
+    code = ("import numpy as np\n"
+            "from hpda_sblas_pnnl_amd import spmv as S\n"
+            "rp = np.array([0, 1], np.int32); col = np.array([0], np.int32)\n"
+            "y = np.zeros(1)\n"
+            "try:\n"
+            "    S.spMV_mgpu_v1(1, 1, 1, 1.0, np.ones(1), rp, col, np.ones(1), 0.0, y, 2, 1)\n"
+            "except Exception as e:\n"
+            "    print('raised', e.status)\n")
+    env = dict(os.environ, PYTHONPATH=ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert r.returncode == 0 and "raised 2" in r.stdout, r.stdout + r.stderr
+    env["SBLAS_EXIT_ON_ERROR"] = "1"
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert r.returncode == 1 and "sblas error 2" in r.stderr, r.stdout + r.stderr
