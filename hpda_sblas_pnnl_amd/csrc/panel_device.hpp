@@ -29,27 +29,36 @@ static_assert(kPanelPad % 2 == 0, "pairs never straddle a slice end");
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-// Two consecutive values (an aligned pair) as doubles, one non-temporal load.
-__device__ __forceinline__ void load2(const double *p, double &a, double &b) {
-  const f64x2 q = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p));
+// An aligned pair of consecutive values as loaded (one non-temporal load),
+// widened to doubles only where the products are formed.  Loads keep their
+// raw registers until then: a conversion right behind a load would make the
+// wave wait for it where the load is issued, ahead of the row sums that the
+// load is meant to overlap.
+template <typename VT> struct RawPair;
+template <> struct RawPair<double> { typedef f64x2 type; };
+template <> struct RawPair<float> { typedef f32x2 type; };
+template <> struct RawPair<uint16_t> { typedef uint32_t type; };  // two bfloat16
+
+template <typename VT>
+__device__ __forceinline__ typename RawPair<VT>::type load_pair(const VT *p) {
+  typedef typename RawPair<VT>::type R;
+  return __builtin_nontemporal_load(reinterpret_cast<const R *>(p));
+}
+__device__ __forceinline__ void widen(f64x2 q, double &a, double &b) {
   a = q.x;
   b = q.y;
 }
-__device__ __forceinline__ void load2(const float *p, double &a, double &b) {
-  const f32x2 q = __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(p));
+__device__ __forceinline__ void widen(f32x2 q, double &a, double &b) {
   a = q.x;
   b = q.y;
 }
-__device__ __forceinline__ void load2(const uint16_t *p, double &a, double &b) {
-  const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+__device__ __forceinline__ void widen(uint32_t w, double &a, double &b) {
   a = static_cast<double>(__uint_as_float(w << 16));
   b = static_cast<double>(__uint_as_float(w & 0xffff0000u));
 }
-// two consecutive 16-bit column offsets
-__device__ __forceinline__ void load2_cols(const uint16_t *p, int &a, int &b) {
-  const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
-  a = static_cast<int>(w & 0xffffu);
-  b = static_cast<int>(w >> 16);
+// two consecutive 16-bit column offsets, one 32-bit load, split at use
+__device__ __forceinline__ uint32_t load_cols(const uint16_t *p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
 }
 
 // Current slice of a row panel and the first element of the chunk worked on;
@@ -77,8 +86,11 @@ __device__ __forceinline__ void seek(Cursor &c, int s, int s1,
 // kLdsDoubles doubles; slots[0] / slots[1] receive the partial sums of a
 // first / last row that the panel's element window cuts (PanelInfo flags).
 // Software-pipelined: the next chunk's values and column offsets are loaded
-// while the current chunk's rows are summed and the next slice's x is staged.
-// The workgroup leaves through a barrier, so LDS may be reused right after.
+// while the current chunk's rows are summed.  Before a slice change the next
+// slice's x entries and row ranges are requested ahead of the chunk's loads:
+// vector loads complete in issue order, so x can then be stored to LDS while
+// the chunk is still on its way.  The workgroup leaves through a barrier, so
+// LDS may be reused right after.
 template <typename VT, int RPT, typename OT>
 __device__ __forceinline__ void row_panel(
     int p, int n, int R, const PanelInfo &pi, const int *__restrict__ slice_base,
@@ -87,42 +99,60 @@ __device__ __forceinline__ void row_panel(
     const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
     double *__restrict__ slots, double *lds) {
   constexpr int W = kPanelCols;
+  constexpr int NX = W / 512;  // x pairs per thread and slice
+  typedef typename RawPair<VT>::type RV;
   double *xs = lds;
   double *prod = lds + W;
   const int t = threadIdx.x;
   const int s0 = slice_base[p], s1 = slice_base[p + 1];
   const int qb = pi.q_lo;
   double acc[RPT];
-  int b[RPT], e[RPT];
-  double v[kItems];
-  int l[kItems];
+  int b[RPT], e[RPT];    // element range [b, e) of row t + 256 j in the current slice
+  int nb[RPT], ne[RPT];  // the same in the next slice
+  RV v[kItems / 2];      // the next chunk's value pairs
+  uint32_t cw[kItems / 2];  // and column offset pairs
+  double x0[NX], x1[NX];    // the next slice's x entries
+  int64_t end2 = 0;  // slice_off at the end of the slice after the current one
 #pragma unroll
   for (int j = 0; j < RPT; j++) acc[j] = 0.0;
 
-  // x[q*W, (q+1)*W) of slice s into LDS, two columns per thread and step
-  // (one 16-byte LDS store); columns past n read as 0
-  auto stage_x = [&](int s) {
+  // the end of the slice after s, read one slice ahead.  Odd threads read
+  // the entry after it, which makes the address per-thread: a vector load,
+  // which the chunk's waits cover, where a scalar load would be waited for at
+  // the next barrier.  The value is taken from a wave's first (even) thread.
+  auto load_end2 = [&](int s) { end2 = slice_off[min(s + 2 + (t & 1), s1)]; };
+  // x[q*W, (q+1)*W) of slice s, two columns per thread and step; the
+  // addresses are clamped (no load under a branch), columns past n are zeroed
+  // when stored (one 16-byte LDS store per step)
+  auto load_x = [&](int s) {
     const int c0 = (qb + (s - s0)) * W;
 #pragma unroll
-    for (int i = 0; i < W / 512; i++) {
+    for (int i = 0; i < NX; i++) {
       const int c = c0 + 2 * t + i * 512;
-      const double x0 = x[min(c, n - 1)], x1 = x[min(c + 1, n - 1)];  // no load under a branch
+      x0[i] = x[min(c, n - 1)];
+      x1[i] = x[min(c + 1, n - 1)];
+    }
+  };
+  auto store_x = [&](int s) {
+    const int c0 = (qb + (s - s0)) * W;
+#pragma unroll
+    for (int i = 0; i < NX; i++) {
+      const int c = c0 + 2 * t + i * 512;
       f64x2 q;
-      q.x = (c < n) ? x0 : 0.0;
-      q.y = (c + 1 < n) ? x1 : 0.0;
+      q.x = (c < n) ? x0[i] : 0.0;
+      q.y = (c + 1 < n) ? x1[i] : 0.0;
       *reinterpret_cast<f64x2 *>(xs + 2 * t + i * 512) = q;
     }
   };
-  // element ranges [b, e) of this thread's rows inside slice s
-  auto row_ranges = [&](int s) {
+  // element ranges of this thread's rows inside slice s; a row past R reads
+  // the slice's end twice (an empty range), with no select behind the load
+  auto load_ranges = [&](int s) {
     const OT *rps = srp + static_cast<int64_t>(s) * (R + 1);
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       const int rl = t + 256 * j;
-      const bool in = rl < R;
-      const int lo = rps[in ? rl : 0], hi = rps[in ? rl + 1 : 0];  // unconditional loads
-      b[j] = in ? lo : 0;
-      e[j] = in ? hi : 0;
+      nb[j] = static_cast<int>(rps[min(rl, R)]);
+      ne[j] = static_cast<int>(rps[min(rl + 1, R)]);
     }
   };
   // values and column offsets of a chunk: pair i of thread t is elements
@@ -133,54 +163,88 @@ __device__ __forceinline__ void row_panel(
 #pragma unroll
     for (int i = 0; i < kItems / 2; i++) {
       const int64_t k = c.base + min(c.k0 + 512 * i + 2 * t, c.len - 2);
-      load2(pval + k, v[2 * i], v[2 * i + 1]);
-      load2_cols(pcol + k, l[2 * i], l[2 * i + 1]);
+      v[i] = load_pair(pval + k);
+      cw[i] = load_cols(pcol + k);
     }
   };
 
   Cursor cur;
   seek(cur, s0, s1, slice_off);
   if (cur.s < s1) {
-    stage_x(cur.s);
-    row_ranges(cur.s);
+    load_end2(cur.s);
+    load_x(cur.s);
+    load_ranges(cur.s);
     load_chunk(cur);
+    store_x(cur.s);
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      b[j] = nb[j];
+      e[j] = ne[j];
+    }
   }
   __syncthreads();
   while (cur.s < s1) {
     // products of the current chunk (x of its slice is in LDS)
 #pragma unroll
     for (int i = 0; i < kItems / 2; i++) {
+      double a0, a1;
+      widen(v[i], a0, a1);
       f64x2 pr;
-      pr.x = mul_rn(v[2 * i], xs[l[2 * i]]);
-      pr.y = mul_rn(v[2 * i + 1], xs[l[2 * i + 1]]);
+      pr.x = mul_rn(a0, xs[cw[i] & 0xffffu]);
+      pr.y = mul_rn(a1, xs[cw[i] >> 16]);
       *reinterpret_cast<f64x2 *>(prod + 512 * i + 2 * t) = pr;  // 16-byte LDS stores
     }
     Cursor nxt = cur;
     nxt.k0 += kChunk;
-    if (nxt.k0 >= cur.len) seek(nxt, cur.s + 1, s1, slice_off);
-    // the next chunk's loads are in flight during the row sums below
-    if (nxt.s < s1) load_chunk(nxt);
-    __syncthreads();  // products visible; x of the current slice no longer read
+    if (nxt.k0 >= cur.len) {  // the next slice starts where this one ends
+      nxt.s = cur.s + 1;
+      nxt.k0 = 0;
+      nxt.base = cur.base + cur.len;
+      if (nxt.s < s1) {
+        const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(end2));
+        const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(end2 >> 32));
+        const int64_t e2 = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                                static_cast<uint32_t>(lo));
+        nxt.len = static_cast<int>(e2 - nxt.base);
+        if (nxt.len <= 0) seek(nxt, nxt.s + 1, s1, slice_off);  // empty slices
+      }
+    }
     const bool new_slice = nxt.s != cur.s && nxt.s < s1;
-    if (new_slice) stage_x(nxt.s);
-    // each row adds its products of this chunk in element order, the next
-    // product's LDS read issued ahead of each add
+    // the next chunk's loads are in flight during the row sums below
+    if (new_slice) {
+      load_end2(nxt.s);
+      load_x(nxt.s);
+      load_ranges(nxt.s);
+      load_chunk(nxt);
+      __syncthreads();  // products visible; x of the current slice no longer read
+      store_x(nxt.s);
+    } else {
+      if (nxt.s < s1) load_chunk(nxt);
+      __syncthreads();
+    }
+    // each row adds its products of this chunk in element order, four LDS
+    // reads issued ahead of four adds; a place past the row's end adds -0.0,
+    // which leaves every sum (signed zeros included) unchanged
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       const int lo = max(b[j], cur.k0) - cur.k0, hi = min(e[j], cur.k0 + kChunk) - cur.k0;
       double a = acc[j];
-      if (lo < hi) {
-        double q = prod[lo];
-        for (int k = lo + 1; k < hi; k++) {
-          const double nq = prod[k];
-          a = add_rn(a, q);
-          q = nq;
-        }
-        a = add_rn(a, q);
+      for (int k = lo; k < hi; k += 4) {
+        double q[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) q[i] = prod[min(k + i, kChunk - 1)];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a = add_rn(a, (k + i < hi) ? q[i] : -0.0);
       }
       acc[j] = a;
     }
-    if (new_slice) row_ranges(nxt.s);
+    if (new_slice) {
+#pragma unroll
+      for (int j = 0; j < RPT; j++) {
+        b[j] = nb[j];
+        e[j] = ne[j];
+      }
+    }
     __syncthreads();  // products consumed; x of the next slice visible
     cur = nxt;
   }
