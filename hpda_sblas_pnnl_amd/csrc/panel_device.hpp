@@ -23,8 +23,37 @@ __device__ __forceinline__ double add_rn(double a, double b) {
 
 constexpr int kItems = 8;               // elements per thread per chunk (4 pairs)
 constexpr int kChunk = 256 * kItems;    // elements per chunk of a slice
-constexpr int kLdsDoubles = kPanelCols + kChunk;  // x panel + products
 static_assert(kPanelPad % 2 == 0, "pairs never straddle a slice end");
+
+// A slice's row offsets are staged in LDS as 32-bit words, in rounds of 256
+// (one per thread): 16-bit offsets as the words covering the slice's R + 1
+// entries (one word more when they start mid-word), 32-bit offsets one word
+// per entry.
+template <typename OT, int RPT>
+__host__ __device__ constexpr int range_rounds() {
+  return sizeof(OT) == 2 ? (128 * RPT + 1 + 255) / 256 : (256 * RPT + 1 + 255) / 256;
+}
+// LDS of one row panel body, in doubles: x panel, products, row offsets.
+// 16-bit offsets with up to 8 rows per thread keep three workgroups per CU.
+template <typename OT, int RPT>
+__host__ __device__ constexpr int lds_doubles() {
+  return kPanelCols + kChunk + range_rounds<OT, RPT>() * 128;
+}
+
+// vmcnt(0) alone (expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// One load of 16 (4) bytes per thread from global memory straight into LDS:
+// the wave's 64 pieces land contiguously at lds_wave, which must be the same
+// for every thread of the wave.  Tracked like a vector load (vmcnt).
+typedef const __attribute__((address_space(1))) void *global_ptr;
+typedef __attribute__((address_space(3))) void *lds_ptr;
+__device__ __forceinline__ void load16_to_lds(const void *src, void *lds_wave) {
+  __builtin_amdgcn_global_load_lds((global_ptr)src, (lds_ptr)lds_wave, 16, 0, 0);
+}
+__device__ __forceinline__ void load4_to_lds(const void *src, void *lds_wave) {
+  __builtin_amdgcn_global_load_lds((global_ptr)src, (lds_ptr)lds_wave, 4, 0, 0);
+}
 
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -83,13 +112,13 @@ __device__ __forceinline__ void seek(Cursor &c, int s, int s1,
 }
 
 // Row panel p of one matrix, by one workgroup of 256 threads.  lds holds
-// kLdsDoubles doubles; slots[0] / slots[1] receive the partial sums of a
-// first / last row that the panel's element window cuts (PanelInfo flags).
-// Software-pipelined: the next chunk's values and column offsets are loaded
-// while the current chunk's rows are summed.  Before a slice change the next
-// slice's x entries and row ranges are requested ahead of the chunk's loads:
-// vector loads complete in issue order, so x can then be stored to LDS while
-// the chunk is still on its way.  The workgroup leaves through a barrier, so
+// lds_doubles<OT, RPT>() doubles; slots[0] / slots[1] receive the partial sums
+// of a first / last row that the panel's element window cuts (PanelInfo
+// flags).  Software-pipelined: the next chunk's values and column offsets are
+// loaded while the current chunk's rows are summed.  At a slice change the
+// next slice's x window and row offsets are loaded straight into LDS (no
+// registers, no stores) once the current slice's products are formed, and
+// land while the rows are summed.  The workgroup leaves through a barrier, so
 // LDS may be reused right after.
 template <typename VT, int RPT, typename OT>
 __device__ __forceinline__ void row_panel(
@@ -99,20 +128,25 @@ __device__ __forceinline__ void row_panel(
     const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
     double *__restrict__ slots, double *lds) {
   constexpr int W = kPanelCols;
-  constexpr int NX = W / 512;  // x pairs per thread and slice
+  constexpr int NR = range_rounds<OT, RPT>();
   typedef typename RawPair<VT>::type RV;
   double *xs = lds;
   double *prod = lds + W;
+  uint32_t *rs = reinterpret_cast<uint32_t *>(lds + W + kChunk);
   const int t = threadIdx.x;
+  const int wv = t & ~63;  // first thread of this thread's wave
   const int s0 = slice_base[p], s1 = slice_base[p + 1];
   const int qb = pi.q_lo;
+  // x in 16-byte pairs when aligned (else in 4-byte pieces)
+  const bool x16 = (reinterpret_cast<uintptr_t>(x) & 15) == 0 && n >= 2;
   double acc[RPT];
-  int b[RPT], e[RPT];    // element range [b, e) of row t + 256 j in the current slice
-  int nb[RPT], ne[RPT];  // the same in the next slice
-  RV v[kItems / 2];      // the next chunk's value pairs
+  int b[RPT], e[RPT];     // element range [b, e) of row t + 256 j in the current slice
+  RV v[kItems / 2];       // the next chunk's value pairs
   uint32_t cw[kItems / 2];  // and column offset pairs
-  double x0[NX], x1[NX];    // the next slice's x entries
-  int64_t end2 = 0;  // slice_off at the end of the slice after the current one
+  uint32_t end2 = 0;      // low word of slice_off at the end of the slice after the current one
+  int r_shift = 0;        // first 16-bit offset of the staged slice inside its first word
+  double xfix = 0.0;      // x[n - 1] when n is odd and this thread's pair holds it
+  int fix_at = -1;
 #pragma unroll
   for (int j = 0; j < RPT; j++) acc[j] = 0.0;
 
@@ -120,39 +154,69 @@ __device__ __forceinline__ void row_panel(
   // the entry after it, which makes the address per-thread: a vector load,
   // which the chunk's waits cover, where a scalar load would be waited for at
   // the next barrier.  The value is taken from a wave's first (even) thread.
-  auto load_end2 = [&](int s) { end2 = slice_off[min(s + 2 + (t & 1), s1)]; };
-  // x[q*W, (q+1)*W) of slice s, two columns per thread and step; the
-  // addresses are clamped (no load under a branch), columns past n are zeroed
-  // when stored (one 16-byte LDS store per step)
-  auto load_x = [&](int s) {
-    const int c0 = (qb + (s - s0)) * W;
-#pragma unroll
-    for (int i = 0; i < NX; i++) {
-      const int c = c0 + 2 * t + i * 512;
-      x0[i] = x[min(c, n - 1)];
-      x1[i] = x[min(c + 1, n - 1)];
-    }
+  auto load_end2 = [&](int s) {
+    end2 = reinterpret_cast<const uint32_t *>(slice_off + min(s + 2 + (t & 1), s1))[0];
   };
-  auto store_x = [&](int s) {
+  // slice s's x window x[q*W, (q+1)*W) and row offsets into LDS.  Pieces of x
+  // past n load a valid address instead (their columns never occur); the pair
+  // (n - 1, n) of an odd n would read past x, so its thread loads x[n - 1]
+  // alone and writes it once the loads have landed.
+  auto request_slice = [&](int s) {
     const int c0 = (qb + (s - s0)) * W;
+    if (x16) {
 #pragma unroll
-    for (int i = 0; i < NX; i++) {
-      const int c = c0 + 2 * t + i * 512;
-      f64x2 q;
-      q.x = (c < n) ? x0[i] : 0.0;
-      q.y = (c + 1 < n) ? x1[i] : 0.0;
-      *reinterpret_cast<f64x2 *>(xs + 2 * t + i * 512) = q;
+      for (int i = 0; i < W / 512; i++) {
+        const int c = c0 + 2 * (i * 256 + t);
+        load16_to_lds(c + 1 < n ? x + c : x, xs + 2 * (i * 256 + wv));
+      }
+      const int d = n - 1 - c0;
+      const bool fix = (n & 1) && d >= 0 && d < W && ((d >> 1) & 255) == t;
+      if (fix) xfix = x[n - 1];
+      fix_at = fix ? d : -1;
+    } else {
+      const uint32_t *xw = reinterpret_cast<const uint32_t *>(x);
+      const int64_t g0 = 2 * static_cast<int64_t>(c0), gmax = 2 * static_cast<int64_t>(n) - 1;
+#pragma unroll 8
+      for (int i = 0; i < 2 * W / 256; i++)
+        load4_to_lds(xw + min(g0 + i * 256 + t, gmax),
+                       reinterpret_cast<uint32_t *>(xs) + i * 256 + wv);
+      fix_at = -1;
     }
+    const int64_t base = static_cast<int64_t>(s) * (R + 1);  // first offset of slice s
+    const uint32_t *src;
+    int cnt;
+    if constexpr (sizeof(OT) == 2) {
+      src = reinterpret_cast<const uint32_t *>(srp) + (base >> 1);
+      cnt = static_cast<int>(((base + R) >> 1) - (base >> 1)) + 1;
+      r_shift = static_cast<int>(base & 1);
+    } else {
+      src = reinterpret_cast<const uint32_t *>(srp) + base;
+      cnt = R + 1;
+      r_shift = 0;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (r * 256 < cnt) load4_to_lds(src + min(r * 256 + t, cnt - 1), rs + r * 256 + wv);
   };
-  // element ranges of this thread's rows inside slice s; a row past R reads
-  // the slice's end twice (an empty range), with no select behind the load
-  auto load_ranges = [&](int s) {
-    const OT *rps = srp + static_cast<int64_t>(s) * (R + 1);
+  // after the requests have landed (this thread's): the odd-n fix-up; after
+  // the barrier that follows: each thread's row ranges, a row past R reading
+  // the slice's end twice (an empty range)
+  auto finish_slice = [&]() {
+    wait_vector_memory();
+    if (fix_at >= 0) xs[fix_at] = xfix;
+  };
+  auto read_ranges = [&]() {
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       const int rl = t + 256 * j;
-      nb[j] = static_cast<int>(rps[min(rl, R)]);
-      ne[j] = static_cast<int>(rps[min(rl + 1, R)]);
+      if constexpr (sizeof(OT) == 2) {
+        const uint16_t *r16 = reinterpret_cast<const uint16_t *>(rs) + r_shift;
+        b[j] = r16[min(rl, R)];
+        e[j] = r16[min(rl + 1, R)];
+      } else {
+        b[j] = static_cast<int>(rs[min(rl, R)]);
+        e[j] = static_cast<int>(rs[min(rl + 1, R)]);
+      }
     }
   };
   // values and column offsets of a chunk: pair i of thread t is elements
@@ -172,17 +236,12 @@ __device__ __forceinline__ void row_panel(
   seek(cur, s0, s1, slice_off);
   if (cur.s < s1) {
     load_end2(cur.s);
-    load_x(cur.s);
-    load_ranges(cur.s);
+    request_slice(cur.s);
     load_chunk(cur);
-    store_x(cur.s);
-#pragma unroll
-    for (int j = 0; j < RPT; j++) {
-      b[j] = nb[j];
-      e[j] = ne[j];
-    }
+    finish_slice();
   }
   __syncthreads();
+  if (cur.s < s1) read_ranges();
   while (cur.s < s1) {
     // products of the current chunk (x of its slice is in LDS)
 #pragma unroll
@@ -201,27 +260,18 @@ __device__ __forceinline__ void row_panel(
       nxt.k0 = 0;
       nxt.base = cur.base + cur.len;
       if (nxt.s < s1) {
-        const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(end2));
-        const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(end2 >> 32));
-        const int64_t e2 = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
-                                                static_cast<uint32_t>(lo));
-        nxt.len = static_cast<int>(e2 - nxt.base);
+        // a slice's length fits an int: the low words' difference is exact
+        const uint32_t e2 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(end2)));
+        nxt.len = static_cast<int>(e2 - static_cast<uint32_t>(nxt.base));
         if (nxt.len <= 0) seek(nxt, nxt.s + 1, s1, slice_off);  // empty slices
       }
     }
     const bool new_slice = nxt.s != cur.s && nxt.s < s1;
     // the next chunk's loads are in flight during the row sums below
-    if (new_slice) {
-      load_end2(nxt.s);
-      load_x(nxt.s);
-      load_ranges(nxt.s);
-      load_chunk(nxt);
-      __syncthreads();  // products visible; x of the current slice no longer read
-      store_x(nxt.s);
-    } else {
-      if (nxt.s < s1) load_chunk(nxt);
-      __syncthreads();
-    }
+    if (nxt.s < s1) load_chunk(nxt);
+    if (new_slice) load_end2(nxt.s);
+    __syncthreads();  // products visible; x and row offsets of this slice no longer read
+    if (new_slice) request_slice(nxt.s);
     // each row adds its products of this chunk in element order, four LDS
     // reads issued ahead of four adds; a place past the row's end adds -0.0,
     // which leaves every sum (signed zeros included) unchanged
@@ -238,14 +288,9 @@ __device__ __forceinline__ void row_panel(
       }
       acc[j] = a;
     }
-    if (new_slice) {
-#pragma unroll
-      for (int j = 0; j < RPT; j++) {
-        b[j] = nb[j];
-        e[j] = ne[j];
-      }
-    }
-    __syncthreads();  // products consumed; x of the next slice visible
+    if (new_slice) finish_slice();
+    __syncthreads();  // products consumed; the next slice's x and row offsets in place
+    if (new_slice) read_ranges();
     cur = nxt;
   }
 #pragma unroll

@@ -247,7 +247,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
                                                  const int *__restrict__ qbound,
                                                  int *__restrict__ ctr) {
   static_assert(BLOCK == 256, "the panel body assumes 256 threads");
-  constexpr int LDS = MODE == kBatchPanel ? panel::kLdsDoubles : BLOCK * ITEMS + kStreamRowsMax;
+  constexpr int LDS = MODE == kBatchPanel ? panel::lds_doubles<uint16_t, kBatchPanelRowsMax / 256>()
+                                          : BLOCK * ITEMS + kStreamRowsMax;
   __shared__ double lds[LDS];
   __shared__ double wsum[BLOCK / 64];
   __shared__ int next;

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_panel_spmv(
     const OT *__restrict__ srp, const VT *__restrict__ pval,
     const uint16_t *__restrict__ pcol, const double *__restrict__ x, double *__restrict__ y,
     double alpha, double beta, double *__restrict__ partials) {
-  __shared__ double lds[panel::kLdsDoubles];
+  __shared__ double lds[panel::lds_doubles<OT, RPT>()];
   int p = blockIdx.x;
   if (xcd_remap) {  // consecutive row panels on one XCD share their x window in its L2
     const int per = gridDim.x >> 3;
@@ -343,8 +343,10 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   SBLAS_HIP_TRY(hipStreamSynchronize(s));
   if (!h_over) {
     // every slice fits 16-bit offsets: keep the narrow copy only
+    // two entries of slack: the kernel stages a slice's offsets as whole
+    // 32-bit words, and the last slice's last word may end past them
     const int64_t cnt_o = ns * (R + 1);
-    f.srp = static_cast<uint16_t *>(alloc(sizeof(uint16_t) * cnt_o));
+    f.srp = static_cast<uint16_t *>(alloc(sizeof(uint16_t) * (cnt_o + 2)));
     if (!f.srp) {
       set_error(SBLAS_ERR_ALLOC, "panel format: allocation failed");
       return SBLAS_ERR_ALLOC;

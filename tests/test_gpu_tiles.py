@@ -223,6 +223,49 @@ def test_panel_form_ragged(oracle):
                           oracle.csr_spmv(m, n, val, rp, col, x, 1.0, 0.0, np.zeros(m)))
 
 
+@pytest.mark.parametrize("case", ["odd_n", "x_offset", "wide_offsets"])
+def test_panel_form_x_and_offsets(oracle, case):
+    """Column-panel form where x ends inside a 16-byte pair (odd n, column
+    n - 1 in use), where x starts 8 bytes past a 16-byte boundary (x staged
+    in 4-byte pieces), and with slices of more than 65535 elements (32-bit
+    row offsets): bitwise equal to the oracle."""
+    import torch
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    rng = np.random.default_rng(53)
+    if case == "wide_offsets":
+        m, n, per = 3000, 5001, 60  # 1344-row panels hold 80640 elements in one column panel
+        col = np.concatenate([np.sort(rng.choice(4096, per, replace=False))
+                              for _ in range(m)]).astype(np.int32)
+        rp = (np.arange(m + 1, dtype=np.int64) * per).astype(np.int32)
+    else:
+        m, n = 9000, 12289
+        lens = rng.integers(0, 40, m)
+        rows = []
+        for i, L in enumerate(lens):
+            c = rng.choice(n - 1, size=L, replace=False)
+            if i % 7 == 0:
+                c = np.append(c, n - 1)
+            rows.append(np.sort(c))
+        col = np.concatenate(rows).astype(np.int32)
+        rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    val = rng.uniform(-1, 1, len(col))
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, m)
+    drp, dcol, dval, dy = _dev(rp, col, val, y0)
+    if case == "x_offset":
+        buf = torch.zeros(n + 1, dtype=torch.float64, device="cuda")
+        buf[1:] = torch.from_numpy(x).cuda()
+        dx = buf[1:]
+        assert dx.data_ptr() % 16 == 8
+    else:
+        (dx,) = _dev(x)
+    plan = SpmvPlan("csr", m, n, drp, dcol, dval, kernel="vector", variant=3)
+    assert plan.info()["variant"] == 3
+    plan.spmv(dx, dy, -1.25, 0.5)
+    torch.cuda.synchronize()
+    assert np.array_equal(dy.cpu().numpy(), oracle.csr_spmv(m, n, val, rp, col, x, -1.25, 0.5, y0))
+
+
 @pytest.mark.parametrize("kind", KINDS)
 def test_panel_form_kinds(oracle, kind):
     """Column-panel form, forced, on rows longer than a chunk, rows ending on
