@@ -141,6 +141,8 @@ __device__ __forceinline__ void row_panel(
   const bool x16 = (reinterpret_cast<uintptr_t>(x) & 15) == 0 && n >= 2;
   double acc[RPT];
   int b[RPT], e[RPT];     // element range [b, e) of row t + 256 j in the current slice
+  int nb[RPT], ne[RPT];   // the same in slice r_next, loaded a whole slice ahead
+  int r_next = -1;
   RV v[kItems / 2];       // the next chunk's value pairs
   uint32_t cw[kItems / 2];  // and column offset pairs
   uint32_t end2 = 0;      // low word of slice_off at the end of the slice after the current one
@@ -161,7 +163,7 @@ __device__ __forceinline__ void row_panel(
   // past n load a valid address instead (their columns never occur); the pair
   // (n - 1, n) of an odd n would read past x, so its thread loads x[n - 1]
   // alone and writes it once the loads have landed.
-  auto request_slice = [&](int s) {
+  auto request_slice = [&](int s, bool ranges) {
     const int c0 = (qb + (s - s0)) * W;
     if (x16) {
 #pragma unroll
@@ -194,9 +196,22 @@ __device__ __forceinline__ void row_panel(
       cnt = R + 1;
       r_shift = 0;
     }
+    if (ranges) {
 #pragma unroll
-    for (int r = 0; r < NR; r++)
-      if (r * 256 < cnt) load4_to_lds(src + min(r * 256 + t, cnt - 1), rs + r * 256 + wv);
+      for (int r = 0; r < NR; r++)
+        if (r * 256 < cnt) load4_to_lds(src + min(r * 256 + t, cnt - 1), rs + r * 256 + wv);
+    }
+  };
+  // row ranges of slice s into registers, a whole slice before they are used
+  auto load_next_ranges = [&](int s) {
+    const OT *rps = srp + static_cast<int64_t>(s) * (R + 1);
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      const int rl = t + 256 * j;
+      nb[j] = static_cast<int>(rps[min(rl, R)]);
+      ne[j] = static_cast<int>(rps[min(rl + 1, R)]);
+    }
+    r_next = s;
   };
   // after the requests have landed (this thread's): the odd-n fix-up; after
   // the barrier that follows: each thread's row ranges, a row past R reading
@@ -236,12 +251,15 @@ __device__ __forceinline__ void row_panel(
   seek(cur, s0, s1, slice_off);
   if (cur.s < s1) {
     load_end2(cur.s);
-    request_slice(cur.s);
+    request_slice(cur.s, true);
     load_chunk(cur);
     finish_slice();
   }
   __syncthreads();
-  if (cur.s < s1) read_ranges();
+  if (cur.s < s1) {
+    read_ranges();
+    if (cur.s + 1 < s1) load_next_ranges(cur.s + 1);
+  }
   while (cur.s < s1) {
     // products of the current chunk (x of its slice is in LDS)
 #pragma unroll
@@ -271,7 +289,7 @@ __device__ __forceinline__ void row_panel(
     if (nxt.s < s1) load_chunk(nxt);
     if (new_slice) load_end2(nxt.s);
     __syncthreads();  // products visible; x and row offsets of this slice no longer read
-    if (new_slice) request_slice(nxt.s);
+    if (new_slice) request_slice(nxt.s, r_next != nxt.s);
     // each row adds its products of this chunk in element order, four LDS
     // reads issued ahead of four adds; a place past the row's end adds -0.0,
     // which leaves every sum (signed zeros included) unchanged
@@ -290,7 +308,18 @@ __device__ __forceinline__ void row_panel(
     }
     if (new_slice) finish_slice();
     __syncthreads();  // products consumed; the next slice's x and row offsets in place
-    if (new_slice) read_ranges();
+    if (new_slice) {
+      if (r_next == nxt.s) {
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+          b[j] = nb[j];
+          e[j] = ne[j];
+        }
+      } else {
+        read_ranges();  // an empty slice lay between: staged with x instead
+      }
+      if (nxt.s + 1 < s1) load_next_ranges(nxt.s + 1);
+    }
     cur = nxt;
   }
 #pragma unroll
