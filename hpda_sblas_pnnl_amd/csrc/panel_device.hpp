@@ -116,10 +116,11 @@ __device__ __forceinline__ void seek(Cursor &c, int s, int s1,
 // of a first / last row that the panel's element window cuts (PanelInfo
 // flags).  Software-pipelined: the next chunk's values and column offsets are
 // loaded while the current chunk's rows are summed.  At a slice change the
-// next slice's x window and row offsets are loaded straight into LDS (no
-// registers, no stores) once the current slice's products are formed, and
-// land while the rows are summed.  The workgroup leaves through a barrier, so
-// LDS may be reused right after.
+// next slice's x window is loaded straight into LDS (no registers, no stores)
+// once the current slice's products are formed, and lands while the rows are
+// summed; its row ranges were loaded into registers at the slice change
+// before (through LDS with x for the first slice and after an empty one).
+// The workgroup leaves through a barrier, so LDS may be reused right after.
 template <typename VT, int RPT, typename OT>
 __device__ __forceinline__ void row_panel(
     int p, int n, int R, const PanelInfo &pi, const int *__restrict__ slice_base,
