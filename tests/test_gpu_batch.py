@@ -81,12 +81,15 @@ def test_batch_panels(oracle, variant, panel_rows):
     """Row panels of the column-panel form mixed with row blocks in the one
     persistent launch (items taken from a device counter that rewinds itself):
     panel-form rows are summed in the host loop's order, so they equal the
-    oracle bit for bit; eager runs and graph replays agree."""
+    oracle bit for bit; eager runs and graph replays agree.  One matrix has
+    an odd n (its last column in use) and its x starts 8 bytes past a 16-byte
+    boundary."""
     import torch
     from hpda_sblas_pnnl_amd import gen as G
     from hpda_sblas_pnnl_amd.device import SpmvBatch
     rng = np.random.default_rng(43)
-    mats = _mats(G) + [("banded_wide", G.banded_csr(60000, seed=6, halfwidth=20000), "f64")]
+    mats = _mats(G) + [("banded_odd", G.banded_csr(30001, seed=7, halfwidth=2000), "f32"),
+                       ("banded_wide", G.banded_csr(60000, seed=6, halfwidth=20000), "f64")]
     items, refs, y0s = [], [], []
     for i, (name, A, vt) in enumerate(mats):
         x = rng.uniform(-1, 1, A.n)
@@ -98,9 +101,16 @@ def test_batch_panels(oracle, variant, panel_rows):
             dv = torch.from_numpy(A.val.astype(np.float32)).cuda()
         else:
             dv = torch.from_numpy(G.to_bf16_bits(A.val).view(np.int16)).cuda()
+        dx = torch.from_numpy(x).cuda()
+        if name == "banded_odd":
+            assert A.n % 2 == 1 and A.col.max() == A.n - 1
+            buf = torch.zeros(A.n + 1, dtype=torch.float64, device="cuda")
+            buf[1:] = dx
+            dx = buf[1:]
+            assert dx.data_ptr() % 16 == 8
         items.append(dict(m=A.m, n=A.n, row_ptr=torch.from_numpy(A.row_ptr).cuda(),
                           col=torch.from_numpy(A.col).cuda(), val=dv,
-                          x=torch.from_numpy(x).cuda(), y=torch.from_numpy(y0.copy()).cuda(),
+                          x=dx, y=torch.from_numpy(y0.copy()).cuda(),
                           alpha=alpha, beta=beta))
         refs.append(_ref(oracle, G, A, vt, x, alpha, beta, y0))
         y0s.append(torch.from_numpy(y0).cuda())
@@ -108,6 +118,7 @@ def test_batch_panels(oracle, variant, panel_rows):
     ch = [batch.choice(i) for i in range(batch.count)]
     if variant == 3:
         assert sum(c["variant"] == 3 for c in ch) >= 3
+        assert ch[len(ch) - 2]["variant"] == 3  # the odd-n matrix
     else:
         assert ch[-1]["variant"] == 3  # the wide band takes column panels by itself
     outs = []
