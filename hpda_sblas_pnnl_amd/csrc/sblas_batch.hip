@@ -239,7 +239,7 @@ __device__ __forceinline__ void rows_typed(int w, const BatchItem &it, const Bat
 
 // One persistent kernel per item kind (MODE kBatchPanel: row panels, else row
 // blocks), so each is compiled with only its own body: its registers and LDS
-// (48 KB for panels, 20-36 KB for row blocks) set its own occupancy.
+// (52 KB for panels, 20-36 KB for row blocks) set its own occupancy.
 template <int MODE, int ITEMS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__restrict__ items,
                                                  const BatchMatrix *__restrict__ mats,

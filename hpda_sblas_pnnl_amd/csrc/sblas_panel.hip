@@ -6,8 +6,8 @@
 // rate, not HBM, bounds a row-parallel kernel.  This form cuts each execution
 // chunk into row panels of up to R rows and the columns into panels of W
 // columns.  A workgroup owns one row panel and walks its column panels in
-// increasing order: it stages x[q*W, (q+1)*W) in LDS with coalesced loads
-// (one pass over the panel's x lines), then streams that slice's elements in
+// increasing order: it loads x[q*W, (q+1)*W) straight into LDS with coalesced
+// loads (one pass over the panel's x lines), then streams that slice's elements in
 // chunks of 2048 (each thread four pairs of consecutive elements, so every
 // load instruction of a wave reads whole lines; slices are padded to 8
 // elements to keep pairs aligned), multiplies each by its x from LDS and
