@@ -182,7 +182,7 @@ __device__ __forceinline__ void row_panel(
 #pragma unroll 8
       for (int i = 0; i < 2 * W / 256; i++)
         load4_to_lds(xw + min(g0 + i * 256 + t, gmax),
-                       reinterpret_cast<uint32_t *>(xs) + i * 256 + wv);
+                     reinterpret_cast<uint32_t *>(xs) + i * 256 + wv);
       fix_at = -1;
     }
     const int64_t base = static_cast<int64_t>(s) * (R + 1);  // first offset of slice s
@@ -291,19 +291,22 @@ __device__ __forceinline__ void row_panel(
     if (new_slice) load_end2(nxt.s);
     __syncthreads();  // products visible; x and row offsets of this slice no longer read
     if (new_slice) request_slice(nxt.s, r_next != nxt.s);
-    // each row adds its products of this chunk in element order, four LDS
-    // reads issued ahead of four adds; a place past the row's end adds -0.0,
-    // which leaves every sum (signed zeros included) unchanged
+    // each row adds its products of this chunk in element order, four at a
+    // time from two 16-byte LDS reads of aligned pairs (the run starts at the
+    // pair holding its first product); a place before the row's first or
+    // past its last product adds -0.0, which leaves every sum (signed zeros
+    // included) unchanged
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       const int lo = max(b[j], cur.k0) - cur.k0, hi = min(e[j], cur.k0 + kChunk) - cur.k0;
       double a = acc[j];
-      for (int k = lo; k < hi; k += 4) {
-        double q[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) q[i] = prod[min(k + i, kChunk - 1)];
-#pragma unroll
-        for (int i = 0; i < 4; i++) a = add_rn(a, (k + i < hi) ? q[i] : -0.0);
+      for (int k = lo & ~1; k < hi; k += 4) {
+        const f64x2 p0 = *reinterpret_cast<const f64x2 *>(prod + k);
+        const f64x2 p1 = *reinterpret_cast<const f64x2 *>(prod + min(k + 2, kChunk - 2));
+        a = add_rn(a, (k >= lo) ? p0.x : -0.0);
+        a = add_rn(a, (k + 1 < hi) ? p0.y : -0.0);
+        a = add_rn(a, (k + 2 < hi) ? p1.x : -0.0);
+        a = add_rn(a, (k + 3 < hi) ? p1.y : -0.0);
       }
       acc[j] = a;
     }
