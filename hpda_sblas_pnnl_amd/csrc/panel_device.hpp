@@ -40,7 +40,10 @@ __host__ __device__ constexpr int lds_doubles() {
   return kPanelCols + kChunk + range_rounds<OT, RPT>() * 128;
 }
 
-// vmcnt(0) alone (expcnt and lgkmcnt left at their maxima)
+// s_waitcnt vmcnt(0) alone.  gfx9-family encoding of the immediate: vmcnt in
+// bits 3:0 and 15:14, expcnt in 6:4, lgkmcnt in 11:8; expcnt (7) and lgkmcnt
+// (15) are left at their maxima, so only vector memory is waited for.  The
+// compiler's own wait insertion takes this wait into account.
 __device__ __forceinline__ void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // One load of 16 (4) bytes per thread from global memory straight into LDS:
