@@ -24,7 +24,7 @@ constexpr size_t kThreadMin = size_t{4} << 20;   // below this one thread copies
 
 void parallel_copy(void *dst, const void *src, size_t bytes) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t T = std::min<size_t>({8, hw, bytes / kThreadMin});
+  const size_t T = std::min<size_t>({16, hw, bytes / kThreadMin});
   if (T <= 1) {
     std::memcpy(dst, src, bytes);
     return;
