@@ -5,7 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <memory>
 #include <vector>
+
+#include "copy_pool.hpp"
 
 namespace sblas {
 
@@ -54,14 +57,15 @@ struct OwnedStream {
 };
 
 // Host-to-device copies of pageable host arrays through a ring of pinned
-// buffers (staging.cpp): host threads fill one slot while earlier slots'
-// transfers run.  to_device issues the transfers on s and returns once the
+// buffers (staging.cpp): a pool of host threads, started once per ring,
+// fills one slot while earlier slots' transfers run.  to_device issues the transfers on s and returns once the
 // last piece is queued; the ring waits for every transfer before it frees
 // its buffers.
 struct PinnedRing {
   std::vector<void *> slots;
   std::vector<hipEvent_t> events;
   std::vector<bool> used;
+  std::unique_ptr<CopyPool> pool;
   size_t next = 0;
   PinnedRing() = default;
   PinnedRing(const PinnedRing &) = delete;

@@ -2,13 +2,13 @@
 // through a ring of pinned buffers.
 //
 // A copy from pageable memory is staged by the runtime at about half the PCIe
-// rate.  Here each copy is cut into pieces; host threads copy a piece into a
+// rate.  Here each copy is cut into pieces; a pool of host threads (started
+// with the ring, so a piece pays no thread start-up) copies a piece into a
 // free pinned slot while earlier slots' DMA transfers run, and a slot is
 // refilled only after the event recorded behind its last transfer has
 // completed.  Transfers go to the stream of the copy, so the kernels that
 // consume them can start while later pieces are still on their way.
 #include <algorithm>
-#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -20,26 +20,7 @@ namespace {
 
 constexpr size_t kSlotBytes = size_t{64} << 20;  // one piece
 constexpr int kSlots = 3;
-constexpr size_t kThreadMin = size_t{4} << 20;   // below this one thread copies
-
-void parallel_copy(void *dst, const void *src, size_t bytes) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t T = std::min<size_t>({16, hw, bytes / kThreadMin});
-  if (T <= 1) {
-    std::memcpy(dst, src, bytes);
-    return;
-  }
-  std::vector<std::thread> th;
-  const size_t per = (bytes + T - 1) / T;
-  for (size_t i = 0; i < T; i++) {
-    const size_t lo = i * per, hi = std::min(bytes, lo + per);
-    if (lo >= hi) break;
-    th.emplace_back([=] {
-      std::memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, hi - lo);
-    });
-  }
-  for (std::thread &t : th) t.join();
-}
+constexpr unsigned kThreadsMax = 16;               // host copy threads (the caller among them)
 
 }  // namespace
 
@@ -53,6 +34,10 @@ PinnedRing::~PinnedRing() {
 
 int PinnedRing::init() {
   if (!slots.empty()) return SBLAS_OK;
+  if (!pool) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    pool.reset(new CopyPool(std::min(kThreadsMax, hw)));
+  }
   for (int k = 0; k < kSlots; k++) {
     void *p = nullptr;
     hipEvent_t e = nullptr;
@@ -80,7 +65,7 @@ int PinnedRing::to_device(void *dst, const void *src, size_t bytes, hipStream_t 
     const size_t len = std::min(kSlotBytes, bytes - off);
     const size_t k = next++ % slots.size();
     if (used[k]) SBLAS_HIP_TRY(hipEventSynchronize(events[k]));  // its last transfer is done
-    parallel_copy(slots[k], static_cast<const char *>(src) + off, len);
+    pool->copy(slots[k], static_cast<const char *>(src) + off, len);
     SBLAS_HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + off, slots[k], len,
                                  hipMemcpyHostToDevice, s));
     SBLAS_HIP_TRY(hipEventRecord(events[k], s));
