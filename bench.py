@@ -447,10 +447,10 @@ def main():
                     help="skip the R-MAT precision and batched measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device copy first (reference for HBM byte counters)")
-    ap.add_argument("--only", choices=["ingest", "stream"],
+    ap.add_argument("--only", choices=["ingest", "stream", "batch"],
                     help="run one side measurement alone and print its JSON (for profiling): "
-                         "device COO ingestion of R-MAT scale 22, or the streamed v2 drop-in "
-                         "on the banded matrix")
+                         "device COO ingestion of R-MAT scale 22, the streamed v2 drop-in "
+                         "on the banded matrix, or the 64-matrix batch")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -472,6 +472,9 @@ def main():
     if args.only == "stream":
         A, x, y0, alpha, beta, _, _, _ = make_workload("banded")
         print(json.dumps({"stream": extras_v2_stream(A, x, y0, alpha, beta, log)}), flush=True)
+        return
+    if args.only == "batch":
+        print(json.dumps({"batched": extras_batched(log)}), flush=True)
         return
     if args.copy_probe:
         # a plain copy of known size: reads 1 GiB, writes 1 GiB

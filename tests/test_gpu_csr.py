@@ -302,8 +302,8 @@ def test_plan_streams_mode_matches_fused(oracle):
     assert np.array_equal(outs[0], outs[1])
     ref = oracle.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, 1.0, 0.0, np.zeros(A.m))
     assert oracle.err_normwise(outs[0], ref) <= TOL
-    # the row-block form (default for fused launches) over the same chunks
-    plan = SpmvPlan("csr", A.m, A.n, rp, col, val, kernel="vector", chunks=8)
+    # the row-block form over the same chunks
+    plan = SpmvPlan("csr", A.m, A.n, rp, col, val, kernel="vector", chunks=8, variant=2)
     assert plan.info()["variant"] == 2
     dy = torch.zeros(A.m, dtype=torch.float64, device="cuda")
     plan.spmv(dx, dy)
