@@ -217,11 +217,20 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     if (kv == 1 && it.m > 0 && it.nnz > 0 && want != 1 && want != 2 &&
         (want == 3 || rs.max_row <= 8192)) {
       const std::vector<ExecChunk> one{ExecChunk{0, it.nnz, 0, it.m}};
-      const int rows = (opts && opts->panel_rows > 0)
-                           ? std::min(opts->panel_rows, kBatchPanelRowsMax)
-                           : kBatchPanelRowsMax;
+      int rows = (opts && opts->panel_rows > 0) ? std::min(opts->panel_rows, kBatchPanelRowsMax)
+                                                : kBatchPanelRowsMax;
       int st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col,
                            it.d_val, one, h_rp.data(), want == 3, 1, f, &panel, nullptr);
+      // slices beyond 16-bit offsets (the batch kernel reads 16-bit ones
+      // only): shorter row panels, down to 192 rows, before giving up
+      while (st == SBLAS_OK && panel && !f.srp && rows > 192 &&
+             !(opts && opts->panel_rows > 0)) {
+        for (void *q : f.buffers) (void)hipFree(q);
+        f = PanelFormat{};
+        rows /= 2;
+        st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col, it.d_val,
+                         one, h_rp.data(), want == 3, 1, f, &panel, nullptr);
+      }
       if (st == SBLAS_OK && panel && !f.srp) panel = false;  // slices beyond 16-bit offsets
       if (st == SBLAS_ERR_UNSUPPORTED && want != 3) {
         clear_error();
