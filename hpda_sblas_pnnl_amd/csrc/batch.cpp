@@ -372,10 +372,14 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // per kind the resident workgroups, never more than there are items
+  // per kind the resident workgroups, never more than there are items.  With
+  // row blocks beside them, panels take one workgroup per CU less than fit:
+  // three panel workgroups fill a CU's LDS, and the row-block launch would
+  // wait for the panel launch to finish (64-matrix batch: 4.87 -> 4.69 ms)
   const int n_rows = static_cast<int>(b->items.size()) - b->n_panel;
-  b->grid_panel = std::min<int>(b->n_panel, std::max(1, cus) *
-                                                batch_occupancy(kBatchPanel, b->items_per_thread));
+  const int occ_panel = batch_occupancy(kBatchPanel, b->items_per_thread);
+  const int panel_per_cu = (n_rows > 0 && occ_panel > 1) ? occ_panel - 1 : occ_panel;
+  b->grid_panel = std::min<int>(b->n_panel, std::max(1, cus) * panel_per_cu);
   b->grid_rows = std::min<int>(n_rows, std::max(1, cus) *
                                            batch_occupancy(kBatchRowBlock, b->items_per_thread));
   if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
