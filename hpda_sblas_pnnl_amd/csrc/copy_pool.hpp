@@ -16,20 +16,20 @@ namespace sblas {
 
 class CopyPool {
  public:
-  // `threads` copies in parallel: the caller and threads - 1 workers
+  // `threads` copies in parallel: the caller and threads - 1 workers.  If
+  // starting a worker fails, the ones started are stopped and joined before
+  // the error propagates.
   explicit CopyPool(unsigned threads) {
-    for (unsigned i = 1; i < std::max(threads, 1u); i++) workers_.emplace_back([this, i] { work(i); });
+    try {
+      for (unsigned i = 1; i < std::max(threads, 1u); i++) workers_.emplace_back([this, i] { work(i); });
+    } catch (...) {
+      stop_all();
+      throw;
+    }
   }
   CopyPool(const CopyPool &) = delete;
   CopyPool &operator=(const CopyPool &) = delete;
-  ~CopyPool() {
-    {
-      std::lock_guard<std::mutex> lk(m_);
-      stop_ = true;
-    }
-    go_.notify_all();
-    for (std::thread &t : workers_) t.join();
-  }
+  ~CopyPool() { stop_all(); }
 
   // dst[0, bytes) = src[0, bytes); returns when every part is copied.  Copies
   // below `serial_below` bytes are done by the caller alone.
@@ -55,6 +55,15 @@ class CopyPool {
   unsigned threads() const { return static_cast<unsigned>(workers_.size()) + 1; }
 
  private:
+  void stop_all() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    go_.notify_all();
+    for (std::thread &t : workers_) t.join();
+    workers_.clear();
+  }
   // part i of n = threads(): a contiguous range, the last one shorter
   void part(unsigned i, char *dst, const char *src, size_t bytes) const {
     const size_t n = threads(), per = (bytes + n - 1) / n;

@@ -9,6 +9,7 @@
 // completed.  Transfers go to the stream of the copy, so the kernels that
 // consume them can start while later pieces are still on their way.
 #include <algorithm>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -34,9 +35,13 @@ PinnedRing::~PinnedRing() {
 
 int PinnedRing::init() {
   if (!slots.empty()) return SBLAS_OK;
-  if (!pool) {
+  if (!pool) {  // no exception leaves the C entry points: without threads, copies run serially
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    pool.reset(new CopyPool(std::min(kThreadsMax, hw)));
+    try {
+      pool.reset(new CopyPool(std::min(kThreadsMax, hw)));
+    } catch (...) {
+      pool.reset();
+    }
   }
   for (int k = 0; k < kSlots; k++) {
     void *p = nullptr;
@@ -65,7 +70,10 @@ int PinnedRing::to_device(void *dst, const void *src, size_t bytes, hipStream_t 
     const size_t len = std::min(kSlotBytes, bytes - off);
     const size_t k = next++ % slots.size();
     if (used[k]) SBLAS_HIP_TRY(hipEventSynchronize(events[k]));  // its last transfer is done
-    pool->copy(slots[k], static_cast<const char *>(src) + off, len);
+    if (pool)
+      pool->copy(slots[k], static_cast<const char *>(src) + off, len);
+    else
+      std::memcpy(slots[k], static_cast<const char *>(src) + off, len);
     SBLAS_HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + off, slots[k], len,
                                  hipMemcpyHostToDevice, s));
     SBLAS_HIP_TRY(hipEventRecord(events[k], s));
