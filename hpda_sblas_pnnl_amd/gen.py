@@ -143,10 +143,21 @@ def coo_rows(A: CSR) -> np.ndarray:
 
 
 def to_bf16_bits(v: np.ndarray) -> np.ndarray:
-    """float64 -> bfloat16 bit patterns (round to nearest even via float32)."""
-    f = np.ascontiguousarray(v, dtype=np.float32).view(np.uint32)
-    rounding = ((f >> 16) & 1) + np.uint32(0x7FFF)
-    return ((f + rounding) >> 16).astype(np.uint16)
+    """float64 -> bfloat16 bit patterns, rounded to nearest (ties to even)
+    straight from the double: the value is cut to 8 significant bits in
+    binary64 (np.rint rounds half to even), which is exact in float32, then
+    the float32's upper 16 bits are taken.  Values below the smallest normal
+    bfloat16 (2^-126) take the float32 path (round-to-nearest-even of the
+    float32 bits), whose double rounding is confined to subnormals."""
+    d = np.ascontiguousarray(v, dtype=np.float64)
+    frac, ex = np.frexp(d)                        # d = frac * 2^ex, |frac| in [0.5, 1)
+    q = np.ldexp(np.rint(np.ldexp(frac, 8)), ex - 8)
+    small = np.abs(d) < np.ldexp(1.0, -126)
+    with np.errstate(over="ignore"):
+        f = q.astype(np.float32).view(np.uint32) >> 16
+    fs = d.astype(np.float32).view(np.uint32)
+    f_small = (fs + ((fs >> 16) & 1) + np.uint32(0x7FFF)) >> 16
+    return np.where(small, f_small, f).astype(np.uint16)
 
 
 def from_bf16_bits(h: np.ndarray) -> np.ndarray:
