@@ -136,13 +136,62 @@ def host_description():
             "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
-def _time_calls(fn, reps=20, warmup=3):
-    """Median wall time (ms) of fn() on the current stream, by torch events
-    around each call (fn launches on torch's current stream)."""
+def gpu_clocks():
+    """The active DPM level of the shader (sclk), memory (mclk) and fabric
+    (fclk) clocks of each amdgpu device in sysfs (the starred line of
+    pp_dpm_*), e.g. {"card1": {"sclk": "2400Mhz", ...}}; empty where the files
+    are not readable."""
+    out = {}
+    base = "/sys/class/drm"
+    try:
+        cards = sorted(d for d in os.listdir(base) if d.startswith("card") and d[4:].isdigit())
+    except OSError:
+        return out
+    for c in cards:
+        e = {}
+        for k in ("sclk", "mclk", "fclk"):
+            try:
+                with open(f"{base}/{c}/device/pp_dpm_{k}") as f:
+                    for line in f:
+                        if "*" in line:
+                            e[k] = line.split(":", 1)[1].replace("*", "").strip()
+            except OSError:
+                pass
+        if e:
+            out[c] = e
+    return out
+
+
+def preheat(fn, floor_ms):
+    """Runs fn back to back (a sync every 50 calls) for at least floor_ms of
+    wall time, so the GPU leaves its idle clock state before the counted
+    warm-up steps.  Returns what it ran."""
     import torch
-    for _ in range(warmup):
-        fn()
-    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    while floor_ms > 0:
+        for _ in range(50):
+            fn()
+        n += 50
+        torch.cuda.synchronize()
+        if (time.perf_counter() - t0) * 1e3 >= floor_ms:
+            break
+    return {"ms": round((time.perf_counter() - t0) * 1e3, 1), "launches": n}
+
+
+def _time_calls(fn, reps=20, warmup=3, preheat_ms=200.0):
+    """Median wall time (ms) of fn() on the current stream, by torch events
+    around each call (fn launches on torch's current stream).  Untimed calls
+    run first for at least preheat_ms, so the GPU is at its working clocks
+    whatever ran before (host-only work lets it drop to idle clocks)."""
+    import torch
+    t0 = time.perf_counter()
+    while True:
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        if (time.perf_counter() - t0) * 1e3 >= preheat_ms:
+            break
     ts = []
     for _ in range(reps):
         e0 = torch.cuda.Event(enable_timing=True)
@@ -214,7 +263,8 @@ def extras_rmat(log_to):
     rp, col, val = (torch.from_numpy(v).to(dev) for v in (A.row_ptr, A.col, A.val))
     dx, dy0 = torch.from_numpy(x).to(dev), torch.from_numpy(y0).to(dev)
     out = {"workload": "R-MAT scale 22, edge factor 17, dedup, binary values, beta 0.5",
-           "m": A.m, "nnz": A.nnz}
+           "m": A.m, "nnz": A.nnz, "cpu_baseline": cpu_sample(A, x, y0, alpha, beta)}
+    log_to(f"[bench] rmat cpu baseline: {out['cpu_baseline']}")
 
     def run(plan, fmt_bytes_val=8, fmt="csr"):
         dy = dy0.clone()
@@ -375,6 +425,94 @@ def extras_banded_precision(A, rp, col, dx, x, y0, alpha, beta, chunks, part, pl
     return out
 
 
+def extras_banded_formats(A, x, y0, alpha, beta, chunks, y_ref, log_to):
+    """The headline matrix as row-sorted COO and as CSC (device arrays, plans
+    with the same chunk count): kernel time per SpMV, GFLOP/s and roofline
+    fraction on each format's own compulsory bytes (COO: a row index per
+    element instead of row pointers; CSC: column pointers and row indices),
+    error against the FP64 oracle."""
+    import torch
+    from hpda_sblas_pnnl_amd import gen as G
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    from oracle import oracle as ora
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    out = {}
+    for fmt in ("coo", "csc"):
+        if fmt == "coo":
+            a0, a1, v = G.coo_rows(A), A.col, A.val
+            cb = compulsory_bytes(A.m, A.n, A.nnz, 8, 1, beta != 0) + A.nnz * 4 - (A.m + 1) * 4
+        else:
+            a0, a1, v = G.to_csc(A)
+            cb = compulsory_bytes(A.m, A.n, A.nnz, 8, 1, beta != 0) + (A.n - A.m) * 4
+        d0, d1, dv = (torch.from_numpy(np.ascontiguousarray(t)).to(dev) for t in (a0, a1, v))
+        p = SpmvPlan(fmt, A.m, A.n, d0, d1, dv, chunks=chunks)
+        dy = torch.from_numpy(y0).to(dev)
+        p.spmv(dx, dy, alpha, beta)
+        torch.cuda.synchronize()
+        err = ora.err_normwise(dy.cpu().numpy(), y_ref)
+        ms = _time_calls(lambda: p.spmv(dx, dy, alpha, beta), reps=20)
+        inf = p.info()
+        out[fmt] = {"ms": round(ms, 4), "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
+                    "GBs": round(cb / ms / 1e6, 1), "frac": round(cb / ms / 1e6 / PEAK_HBM_GBS, 4),
+                    "err_normwise": err, "kernel": inf["kernel"], "variant": inf["variant"],
+                    "chunks": chunks}
+        log_to(f"[bench] banded {fmt}: {out[fmt]}")
+        del p, d0, d1, dv, dy
+        torch.cuda.empty_cache()
+    return out
+
+
+def cpu_sample(A, x, y0, alpha, beta, reps=10):
+    """The oracle's row loop with OpenMP over rows on this host's cores:
+    median of reps passes over the whole matrix (GFLOP/s), and one serial pass."""
+    from oracle import oracle as ora
+    cores = len(os.sched_getaffinity(0))
+    ora.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, alpha, beta, y0, threads=cores)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ora.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, alpha, beta, y0, threads=cores)
+        ts.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    ora.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, alpha, beta, y0)
+    t1 = time.perf_counter() - t0
+    t = float(np.median(ts))
+    return {"value": round(2 * A.nnz / t / 1e9, 3), "unit": "GFLOP/s", "cores": cores,
+            "kind": "port", "sample": f"whole matrix, median of {reps} OpenMP passes "
+                                      f"({t * 1e3:.2f} ms each)",
+            "serial_gflops": round(2 * A.nnz / t1 / 1e9, 3)}
+
+
+def extras_powerlaw(log_to):
+    """The n = 20,000 power-law matrix (alpha 1, beta 0), kernel chosen by the
+    autotuner: time per SpMV (launch-bound: 4 MB of matrix stays in L2 / MALL)
+    and the CPU baseline on the same matrix."""
+    import torch
+    from hpda_sblas_pnnl_amd import gen as G
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    from oracle import oracle as ora
+    dev = torch.device("cuda:0")
+    A = G.powerlaw_csr(20000, seed=0)
+    x = G.values(A.n, 0, 2, "uniform")
+    y0 = np.zeros(A.m)
+    ref = ora.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, 1.0, 0.0, y0)
+    rp, col, val = (torch.from_numpy(v).to(dev) for v in (A.row_ptr, A.col, A.val))
+    dx, dy = torch.from_numpy(x).to(dev), torch.from_numpy(y0).to(dev)
+    p = SpmvPlan("csr", A.m, A.n, rp, col, val)
+    p.spmv(dx, dy, 1.0, 0.0)
+    torch.cuda.synchronize()
+    err = ora.err_normwise(dy.cpu().numpy(), ref)
+    ms = _time_calls(lambda: p.spmv(dx, dy, 1.0, 0.0), reps=200)
+    inf = p.info()
+    out = {"workload": "power-law n=20,000, p(L)~(L+2.23)^-2 on [1,2000], FP64 CSR",
+           "nnz": A.nnz, "ms": round(ms, 4), "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
+           "err_normwise": err, "kernel": inf["kernel"], "variant": inf["variant"],
+           "cpu_baseline": cpu_sample(A, x, y0, 1.0, 0.0, reps=50)}
+    log_to(f"[bench] powerlaw: {out}")
+    return out
+
+
 def extras_dropin(A, x, y0, alpha, beta, log_to):
     """The reference's own entry point, spMV_mgpu_v1_numa (ngpu = 1, kernel 1),
     on host arrays: every call uploads the matrix, runs, and downloads y, so
@@ -422,12 +560,36 @@ def extras_v2_stream(A, x, y0, alpha, beta, log_to):
     return out
 
 
+# Sources of the column-panel kernel: the measured HBM traffic recorded in
+# profiles/traffic_<workload>.json holds their hash, and a changed kernel makes
+# the recorded figure stale.
+KERNEL_SOURCES = ["hpda_sblas_pnnl_amd/csrc/sblas_panel.hip",
+                  "hpda_sblas_pnnl_amd/csrc/panel_device.hpp",
+                  "hpda_sblas_pnnl_amd/csrc/internal.hpp"]
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def load_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    runs (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel's readings), or None
+    when no record exists or the kernel's sources changed since it was taken."""
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f).get("traffic_bytes")
-    return None
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        rec = json.load(f)
+    if rec.get("source_hash") != kernel_source_hash():
+        log(f"[bench] {p}: recorded for other kernel sources; traffic not reported")
+        return None
+    return rec.get("traffic_bytes")
 
 
 def main():
@@ -442,6 +604,8 @@ def main():
     ap.add_argument("--items", type=int, default=0)
     ap.add_argument("--subwave", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=-1)
+    ap.add_argument("--preheat-ms", type=float, default=500.0,
+                    help="untimed launches for at least this long before the warm-up steps")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the R-MAT precision and batched measurements")
@@ -506,24 +670,33 @@ def main():
     torch.cuda.synchronize()
     y_gpu = dy.cpu().numpy()
 
+    clocks_idle = gpu_clocks()
+    heat = preheat(lambda: plan.spmv(dx, dy, alpha, beta), args.preheat_ms)
     for _ in range(args.warmup):
         plan.spmv(dx, dy, alpha, beta)
     torch.cuda.synchronize()
+    clocks_before = gpu_clocks()
     plan.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.spmv(dx, dy, alpha, beta)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    clocks_after = gpu_clocks()
     ktimes = plan.kernel_times_ms()
     plan.set_timing(False)
-    log(f"[bench] {args.steps} steps in {wall * 1e3:.2f} ms")
+    log(f"[bench] preheat {heat}; {args.steps} steps in {wall * 1e3:.2f} ms; clocks "
+        f"idle {clocks_idle} before {clocks_before} after {clocks_after}")
 
     ms_step = wall * 1e3 / args.steps
     gflops = 2.0 * A.nnz / (ms_step * 1e-3) / 1e9
     cb = compulsory_bytes(A.m, A.n, A.nnz, 8, 1, beta != 0)
     k_ms = float(np.mean(ktimes)) if len(ktimes) else float("nan")
     achieved = cb / (k_ms * 1e-3) / 1e9
+    kstats = ({"min": round(float(np.min(ktimes)), 4), "median": round(float(np.median(ktimes)), 4),
+               "max": round(float(np.max(ktimes)), 4), "launches": int(len(ktimes))}
+              if len(ktimes) else {})
+    log(f"[bench] kernel ms per launch: mean {k_ms:.4f} {kstats}")
     line = {
         "metric": "FP64 SpMV GFLOP/s + % HBM roofline",
         "value": round(gflops, 2),
@@ -545,7 +718,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": load_traffic(args.workload),
-                     "algorithmic_bytes": cb, "kernel_ms": round(k_ms, 4)},
+                     "algorithmic_bytes": cb, "kernel_ms": round(k_ms, 4),
+                     "kernel_ms_stats": kstats},
+        # untimed: launches run before the counted warm-up steps so the GPU
+        # leaves its idle clocks; clocks as sysfs reports them around the loop
+        "preheat": heat,
+        "gpu_clocks": {"before": clocks_before, "after": clocks_after},
         # the CSR5 library's own byte model (include/detail/utils.h:10-14), which
         # reads x once per element: an upper-bound view, not the roofline
         "csr5_model": {"bytes": csr5_model_bytes(A.m, A.nnz),
@@ -568,6 +746,16 @@ def main():
             extras["banded_precision"] = {"error": f"{type(e).__name__}: {e}"}
         del plan, rp, col, val
         torch.cuda.empty_cache()
+        try:
+            extras["banded_formats"] = extras_banded_formats(A, x, y0, alpha, beta, chunks, y_ref,
+                                                             log)
+        except Exception as e:  # noqa: BLE001
+            extras["banded_formats"] = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
+        try:
+            extras["powerlaw"] = extras_powerlaw(log)
+        except Exception as e:  # noqa: BLE001
+            extras["powerlaw"] = {"error": f"{type(e).__name__}: {e}"}
         try:
             extras["dropin_host_arrays"] = extras_dropin(A, x, y0, alpha, beta, log)
         except Exception as e:  # noqa: BLE001
