@@ -63,7 +63,8 @@ assert CHUNK_DTYPE.itemsize == ctypes.sizeof(ChunkDesc) == 48
 class Opts(ctypes.Structure):
     _fields_ = [("kernel", _I), ("chunks", _I), ("part_mode", _I), ("nb", _I),
                 ("streams", _I), ("subwave", _I), ("sigma", _I), ("xcd_remap", _I),
-                ("variant", _I), ("items", _I), ("panel_rows", _I), ("reserved", _I * 5)]
+                ("variant", _I), ("items", _I), ("panel_rows", _I), ("narrow_x", _I),
+                ("reserved", _I * 4)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -72,10 +73,13 @@ class PlanInfo(ctypes.Structure):
                 ("m", _I64), ("n", _I64), ("nnz", _I64), ("workspace_bytes", _I64),
                 ("mean_row", _D), ("cv_row", _D), ("max_row", _I64), ("launches", _I),
                 ("variant", _I), ("row_blocks", _I), ("items", _I),
-                ("panel_rows", _I), ("panel_slices", _I), ("reserved", _I * 2)]
+                ("panel_rows", _I), ("panel_slices", _I), ("tuned", _I),
+                ("trial_ms", ctypes.c_float * 3)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "trial_ms"}
+        d["trial_ms"] = [round(float(v), 4) for v in self.trial_ms]
+        return d
 
 
 # Every function include/sblas.h declares, with its ctypes signature.
@@ -153,10 +157,10 @@ class BatchItem(ctypes.Structure):
 class BatchChoice(ctypes.Structure):
     _fields_ = [("kernel", _I), ("subwave", _I), ("items", _I), ("chunks", _I),
                 ("row_blocks", _I), ("variant", _I), ("mean_row", _D), ("cv_row", _D),
-                ("max_row", _I64)]
+                ("max_row", _I64), ("panel_rows", _I), ("reserved_", _I)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved_"}
 
 _lib = None
 _gen = None

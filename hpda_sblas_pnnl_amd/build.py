@@ -29,6 +29,7 @@ HIP_SOURCES = [
     "sblas_csr5.hip",
     "sblas_batch.hip",
     "sblas_panel.hip",
+    "sblas_sorted.hip",
     "sblas_convert.hip",
     "batch.cpp",
     "ingest.cpp",

@@ -256,7 +256,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     b->mats.push_back(M);
     if (it.m == 0) {
       b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks, 0,
-                                              2, rs.mean, rs.cv, rs.max_row});
+                                              2, rs.mean, rs.cv, rs.max_row, 0, 0});
       continue;
     }
     std::vector<Pending> &mine = pending[i];
@@ -278,7 +278,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     }
     b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks,
                                             static_cast<int>(mine.size()), panel ? 3 : 2, rs.mean,
-                                            rs.cv, rs.max_row});
+                                            rs.cv, rs.max_row, panel ? f.rows : 0, 0});
     h_rps[i] = std::move(h_rp);
   }
   // list order: the row-panel items, then the row-block items (one persistent

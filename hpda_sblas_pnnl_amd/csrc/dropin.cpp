@@ -108,8 +108,10 @@ spmv_ret run_format(int format, int m, int n, int nnz, const double *alpha, cons
   }
   EventPair up, comp, merge, down;
   {
-    // uploads through pinned staging buffers (about twice the pageable rate)
-    PinnedRing ring;
+    // uploads through the process-wide pinned staging ring (about twice the
+    // pageable rate); its one-time set-up happens before the timed uploads
+    StagingLease ring(sizeof(int) * a0_len + (sizeof(int) + sizeof(double)) * nnz +
+                      sizeof(double) * (static_cast<size_t>(n) + m));
     (void)hipEventRecord(up.a, s);
     const bool ok = ring.to_device(d_a0, a0, sizeof(int) * a0_len, s) == SBLAS_OK &&
                     ring.to_device(d_col, a1, sizeof(int) * nnz, s) == SBLAS_OK &&
@@ -409,9 +411,10 @@ spmv_ret spMV_mgpu_baseline_coo(int m, int n, int nnz, double *alpha, double *co
                                 double *y, int ngpu) {
   if (!check_common(m, n, nnz, alpha, beta, cooVal, cooRowIdx, cooColIdx, x, y, ngpu))
     return nan_ret();
-  // COO consumed directly by the segmented-reduction kernel (no conversion)
+  // as the reference: row pointers from the sorted row indices (on the
+  // device), then kernel 1 (src/dspmv_mgpu_baseline_coo.cu:146-165)
   return run_format(SBLAS_FMT_COO, m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y,
-                    SBLAS_KERNEL_COO, SBLAS_PART_ROWBLOCK, 0, 1, 0.0);
+                    SBLAS_KERNEL_VECTOR, SBLAS_PART_ROWBLOCK, 0, 1, 0.0);
 }
 
 spmv_ret spMV_mgpu_v1_numa_coo(int m, int n, int nnz, double *alpha, double *cooVal,
@@ -422,8 +425,11 @@ spmv_ret spMV_mgpu_v1_numa_coo(int m, int n, int nnz, double *alpha, double *coo
   if (!check_common(m, n, nnz, alpha, beta, cooVal, cooRowIdx, cooColIdx, x, y, ngpu) ||
       !check_kernel(kernel))
     return nan_ret();
+  // row pointers from the sorted row indices (on the device), then the
+  // kernel asked for (the reference runs kernel 1 whatever kernel says,
+  // src/dspmv_mgpu_v1_numa_coo.cu:472)
   return run_format(SBLAS_FMT_COO, m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y,
-                    SBLAS_KERNEL_COO, SBLAS_PART_V1_NUMA, 0, 1, 0.0);
+                    kernel, SBLAS_PART_V1_NUMA, 0, 1, 0.0);
 }
 
 spmv_ret spMV_mgpu_baseline_csc(int m, int n, int nnz, double *alpha, double *cscVal,

@@ -58,9 +58,9 @@ struct OwnedStream {
 
 // Host-to-device copies of pageable host arrays through a ring of pinned
 // buffers (staging.cpp): a pool of host threads, started once per ring,
-// fills one slot while earlier slots' transfers run.  to_device issues the transfers on s and returns once the
-// last piece is queued; the ring waits for every transfer before it frees
-// its buffers.
+// fills one slot while earlier slots' transfers run.  to_device issues the
+// transfers on s and returns once the last piece is queued; a slot is
+// refilled only after its previous transfer has completed.
 struct PinnedRing {
   std::vector<void *> slots;
   std::vector<hipEvent_t> events;
@@ -73,6 +73,30 @@ struct PinnedRing {
   ~PinnedRing();
   int init();
   int to_device(void *dst, const void *src, size_t bytes, hipStream_t s);
+  int wait_all();  // every transfer issued so far has completed
+};
+
+// One call's use of the process-wide staging ring of the current device.
+// The ring (3 x 64 MB pinned, a pool of copy threads) is created by the first
+// call that stages and kept until the process ends.  A call stages only when
+// its uploads total at least kStageMinBytes, the ring is free (another
+// thread's call holding it makes this one copy directly) and the pinned
+// allocation succeeds; otherwise to_device copies straight from the pageable
+// array.  The destructor waits for this call's staged transfers before
+// handing the ring on.
+class StagingLease {
+ public:
+  static constexpr size_t kStageMinBytes = size_t{8} << 20;
+  explicit StagingLease(size_t total_bytes);
+  ~StagingLease();
+  StagingLease(const StagingLease &) = delete;
+  StagingLease &operator=(const StagingLease &) = delete;
+  int to_device(void *dst, const void *src, size_t bytes, hipStream_t s);
+  bool staged() const { return ring_ != nullptr; }
+
+ private:
+  PinnedRing *ring_ = nullptr;
+  int device_ = -1;
 };
 
 }  // namespace sblas

@@ -15,6 +15,7 @@ namespace sblas {
 // ---- status -----------------------------------------------------------------
 void set_error(int status, const std::string &what);
 int last_error();
+void clear_error();
 
 #define SBLAS_HIP_TRY(expr)                                                    \
   do {                                                                         \
@@ -68,6 +69,30 @@ struct StreamLaunch {
   int grid;
   int xcd_remap;
 };
+
+// Column-sorted row blocks (sblas_sorted.hip): each block's elements stored
+// sorted by column with their offsets in the block's window.
+constexpr int kSortedBlock = 512;   // threads per block
+constexpr int kSortedItems = 16;    // elements per thread
+constexpr int kSortedCap = kSortedBlock * kSortedItems;
+struct SortedFormat {
+  int nblocks = 0, grid = 0, subwave = 4, xcd_remap = 1;
+  RowBlock *blocks = nullptr;   // device copy of hblocks
+  int *scol = nullptr;          // columns, sorted within each block's window
+  void *sval = nullptr;         // values in the same order
+  uint16_t *spos = nullptr;     // each element's offset in its block's window
+  double *partials = nullptr;   // 2 slots per block
+  std::vector<RowBlock> hblocks;
+  std::vector<void *> buffers;
+  int64_t bytes = 0;
+};
+// Builds the format on the device from the caller's CSR (blocks of at most
+// kSortedCap elements and kStreamRowsMax rows).
+int sorted_build(int val_type, int n, const int *col, const void *val,
+                 const std::vector<RowBlock> &hblocks, int subwave, int xcd_remap,
+                 SortedFormat &f);
+int launch_csr_sorted(int val_type, const SortedFormat &f, const int *rp, const double *x,
+                      double *y, double alpha, double beta, hipStream_t s);
 
 int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *blocks,
                       const int *rp, const int *col, const void *val, const double *x,
@@ -173,13 +198,16 @@ struct PanelFormat {
   void *pval = nullptr;         // values, slice-major, slices padded to kPanelPad
   uint16_t *pcol = nullptr;     // column offsets inside the column panel (padding: 0)
   double *partials = nullptr;   // 2 slots per row panel
+  // narrow-x form (FP32 / BF16 values): FP32 copy of x written before each
+  // launch, padded with zeros to whole column panels; nullptr: FP64 x
+  float *x32 = nullptr;
   int64_t padded = 0;           // elements stored, slices padded
   std::vector<ExecChunk> hpanels;  // host copy: element window and rows of each panel
   std::vector<void *> buffers;
 };
 int panel_rpt(int rows);
 // Rows per row panel so the panels fill whole rounds of resident workgroups.
-int panel_rows_for(int val_type, int m, int device);
+int panel_rows_for(int val_type, int m, int device, bool narrow_x = false);
 // Builds the format on the device: each execution chunk of `ex` cut into row
 // panels of at most `rows` rows (h_rp: host copy of the row pointers).  With
 // force == false it stops after the per-panel column ranges and sets

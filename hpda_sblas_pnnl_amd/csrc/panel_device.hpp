@@ -35,9 +35,10 @@ __host__ __device__ constexpr int range_rounds() {
 }
 // LDS of one row panel body, in doubles: x panel, products, row offsets.
 // 16-bit offsets with up to 8 rows per thread keep three workgroups per CU.
-template <typename OT, int RPT>
+// With NX (narrow x: FP32 x window and FP32 products) the first two halve.
+template <typename OT, int RPT, bool NX = false>
 __host__ __device__ constexpr int lds_doubles() {
-  return kPanelCols + kChunk + range_rounds<OT, RPT>() * 128;
+  return (NX ? (kPanelCols + kChunk) / 2 : kPanelCols + kChunk) + range_rounds<OT, RPT>() * 128;
 }
 
 // s_waitcnt vmcnt(0) alone.  gfx9-family encoding of the immediate: vmcnt in
@@ -60,6 +61,7 @@ __device__ __forceinline__ void load4_to_lds(const void *src, void *lds_wave) {
 
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // An aligned pair of consecutive values as loaded (one non-temporal load),
 // widened to doubles only where the products are formed.  Loads keep their
@@ -87,6 +89,23 @@ __device__ __forceinline__ void widen(f32x2 q, double &a, double &b) {
 __device__ __forceinline__ void widen(uint32_t w, double &a, double &b) {
   a = static_cast<double>(__uint_as_float(w << 16));
   b = static_cast<double>(__uint_as_float(w & 0xffff0000u));
+}
+// the same pairs as FP32 (exact for FP32 and BF16 values)
+__device__ __forceinline__ void widen_f(f32x2 q, float &a, float &b) {
+  a = q.x;
+  b = q.y;
+}
+__device__ __forceinline__ void widen_f(uint32_t w, float &a, float &b) {
+  a = __uint_as_float(w << 16);
+  b = __uint_as_float(w & 0xffff0000u);
+}
+__device__ __forceinline__ void widen_f(f64x2 q, float &a, float &b) {
+  a = static_cast<float>(q.x);
+  b = static_cast<float>(q.y);
+}
+__device__ __forceinline__ float mul_rn_f(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
 }
 // two consecutive 16-bit column offsets, one 32-bit load, split at use
 __device__ __forceinline__ uint32_t load_cols(const uint16_t *p) {
@@ -124,19 +143,29 @@ __device__ __forceinline__ void seek(Cursor &c, int s, int s1,
 // summed; its row ranges were loaded into registers at the slice change
 // before (through LDS with x for the first slice and after an empty one).
 // The workgroup leaves through a barrier, so LDS may be reused right after.
-template <typename VT, int RPT, typename OT>
+//
+// NX (narrow x, for FP32 / BF16 values): x is staged from an FP32 copy
+// (x32, padded with zeros to whole column panels), products are rounded to
+// FP32 and kept as FP32 in LDS, and each row's products of a chunk are added
+// in FP64 into two accumulators (even and odd places) that join the row's sum
+// once per chunk.  Half the LDS traffic per element and half the dependent
+// adds, at an error of a few FP32 roundings per product (x and the product)
+// instead of bit-exact sums.
+template <typename VT, int RPT, typename OT, bool NX = false>
 __device__ __forceinline__ void row_panel(
     int p, int n, int R, const PanelInfo &pi, const int *__restrict__ slice_base,
     const int64_t *__restrict__ slice_off, const OT *__restrict__ srp,
     const VT *__restrict__ pval, const uint16_t *__restrict__ pcol,
     const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
-    double *__restrict__ slots, double *lds) {
+    double *__restrict__ slots, double *lds, const float *__restrict__ x32 = nullptr) {
   constexpr int W = kPanelCols;
   constexpr int NR = range_rounds<OT, RPT>();
   typedef typename RawPair<VT>::type RV;
   double *xs = lds;
   double *prod = lds + W;
-  uint32_t *rs = reinterpret_cast<uint32_t *>(lds + W + kChunk);
+  float *xsf = reinterpret_cast<float *>(lds);
+  float *prodf = reinterpret_cast<float *>(lds + W / 2);
+  uint32_t *rs = reinterpret_cast<uint32_t *>(lds + (NX ? (W + kChunk) / 2 : W + kChunk));
   const int t = threadIdx.x;
   const int wv = t & ~63;  // first thread of this thread's wave
   const int s0 = slice_base[p], s1 = slice_base[p + 1];
@@ -169,7 +198,13 @@ __device__ __forceinline__ void row_panel(
   // alone and writes it once the loads have landed.
   auto request_slice = [&](int s, bool ranges) {
     const int c0 = (qb + (s - s0)) * W;
-    if (x16) {
+    if constexpr (NX) {
+      // four FP32 x per 16-byte piece; x32 is padded to whole panels
+#pragma unroll
+      for (int i = 0; i < W / 1024; i++)
+        load16_to_lds(x32 + c0 + 4 * (i * 256 + t), xsf + 4 * (i * 256 + wv));
+      fix_at = -1;
+    } else if (x16) {
 #pragma unroll
       for (int i = 0; i < W / 512; i++) {
         const int c = c0 + 2 * (i * 256 + t);
@@ -268,12 +303,21 @@ __device__ __forceinline__ void row_panel(
     // products of the current chunk (x of its slice is in LDS)
 #pragma unroll
     for (int i = 0; i < kItems / 2; i++) {
-      double a0, a1;
-      widen(v[i], a0, a1);
-      f64x2 pr;
-      pr.x = mul_rn(a0, xs[cw[i] & 0xffffu]);
-      pr.y = mul_rn(a1, xs[cw[i] >> 16]);
-      *reinterpret_cast<f64x2 *>(prod + 512 * i + 2 * t) = pr;  // 16-byte LDS stores
+      if constexpr (NX) {
+        float a0, a1;
+        widen_f(v[i], a0, a1);
+        f32x2 pr;
+        pr.x = mul_rn_f(a0, xsf[cw[i] & 0xffffu]);
+        pr.y = mul_rn_f(a1, xsf[cw[i] >> 16]);
+        *reinterpret_cast<f32x2 *>(prodf + 512 * i + 2 * t) = pr;  // 8-byte LDS stores
+      } else {
+        double a0, a1;
+        widen(v[i], a0, a1);
+        f64x2 pr;
+        pr.x = mul_rn(a0, xs[cw[i] & 0xffffu]);
+        pr.y = mul_rn(a1, xs[cw[i] >> 16]);
+        *reinterpret_cast<f64x2 *>(prod + 512 * i + 2 * t) = pr;  // 16-byte LDS stores
+      }
     }
     Cursor nxt = cur;
     nxt.k0 += kChunk;
@@ -302,16 +346,30 @@ __device__ __forceinline__ void row_panel(
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       const int lo = max(b[j], cur.k0) - cur.k0, hi = min(e[j], cur.k0 + kChunk) - cur.k0;
-      double a = acc[j];
-      for (int k = lo & ~1; k < hi; k += 4) {
-        const f64x2 p0 = *reinterpret_cast<const f64x2 *>(prod + k);
-        const f64x2 p1 = *reinterpret_cast<const f64x2 *>(prod + min(k + 2, kChunk - 2));
-        a = add_rn(a, (k >= lo) ? p0.x : -0.0);
-        a = add_rn(a, (k + 1 < hi) ? p0.y : -0.0);
-        a = add_rn(a, (k + 2 < hi) ? p1.x : -0.0);
-        a = add_rn(a, (k + 3 < hi) ? p1.y : -0.0);
+      if constexpr (NX) {
+        // aligned runs of four FP32 products (one 16-byte LDS read), even
+        // and odd places into two FP64 sums
+        double a0 = 0.0, a1 = 0.0;
+        for (int k = lo & ~3; k < hi; k += 4) {
+          const f32x4 q = *reinterpret_cast<const f32x4 *>(prodf + k);
+          a0 += static_cast<double>((k >= lo) ? q.x : -0.0f);
+          a1 += static_cast<double>((k + 1 >= lo && k + 1 < hi) ? q.y : -0.0f);
+          a0 += static_cast<double>((k + 2 >= lo && k + 2 < hi) ? q.z : -0.0f);
+          a1 += static_cast<double>((k + 3 < hi) ? q.w : -0.0f);
+        }
+        acc[j] += a0 + a1;
+      } else {
+        double a = acc[j];
+        for (int k = lo & ~1; k < hi; k += 4) {
+          const f64x2 p0 = *reinterpret_cast<const f64x2 *>(prod + k);
+          const f64x2 p1 = *reinterpret_cast<const f64x2 *>(prod + min(k + 2, kChunk - 2));
+          a = add_rn(a, (k >= lo) ? p0.x : -0.0);
+          a = add_rn(a, (k + 1 < hi) ? p0.y : -0.0);
+          a = add_rn(a, (k + 2 < hi) ? p1.x : -0.0);
+          a = add_rn(a, (k + 3 < hi) ? p1.y : -0.0);
+        }
+        acc[j] = a;
       }
-      acc[j] = a;
     }
     if (new_slice) finish_slice();
     __syncthreads();  // products consumed; the next slice's x and row offsets in place

@@ -10,11 +10,14 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 
 namespace sblas {
@@ -71,33 +74,54 @@ RowStats row_stats(int m, const int *rp) {
   return s;
 }
 
-// Kernel choice from row-length statistics.  The row-block form of kernel 1
-// packs short rows and cuts long rows into block-sized pieces, so it stays
-// balanced on heavy-tailed matrices too; measured on MI355X it beats the
-// merge path on R-MAT (0.51 vs 0.76 ms at scale 22) and on the banded
-// matrix, so it is the default for every row-length profile.
-int autotune_kernel(const RowStats &s) {
-  (void)s;
-  return SBLAS_KERNEL_VECTOR;
-}
-
 // Lanes per row: about three quarters of the mean row length, so most rows
 // finish in one or two strides of the sub-wave.
 int autotune_subwave(const RowStats &s) { return pow2_round(s.mean * 0.75, 2, 64); }
 
-// CSR5-style tile depth: the reference's rule (include/anonymouslib_cuda.h:
-// 294-318) gives sigma in [4, 32] from nnz/m; tiles here are 64 lanes wide,
-// and sigma is kept in [8, 32] so each lane has several elements in flight.
-int autotune_sigma(const RowStats &s) {
-  const int per_row = static_cast<int>(s.mean);
-  int sigma = per_row <= 4 ? 4 : (per_row <= 32 ? per_row : (per_row <= 256 ? 32 : 6));
-  return std::max(8, std::min(32, sigma));
+// CSR5-style tile depth: the reference's rule on integer nnz / m
+// (include/anonymouslib_cuda.h:294-318: <= 4 -> 4, <= 32 -> nnz/m, <= 256 ->
+// 32, else 6 elements per lane), snapped to the depths this kernel is built
+// for (8, 16, 32), nearest in log2.
+int autotune_sigma(int64_t nnz, int m) {
+  const int64_t r = m > 0 ? nnz / m : 0;
+  const int ref = r <= 4 ? 4 : (r <= 32 ? static_cast<int>(r) : (r <= 256 ? 32 : 6));
+  return ref <= 11 ? 8 : (ref <= 22 ? 16 : 32);
 }
 
+namespace {
+
+// Cached decisions of the timed trial, keyed by the row-length profile.
+struct TuneKey {
+  int val_type, mean_q, cv_q, empty_q, nnz_q;
+  bool operator<(const TuneKey &o) const {
+    if (val_type != o.val_type) return val_type < o.val_type;
+    if (mean_q != o.mean_q) return mean_q < o.mean_q;
+    if (cv_q != o.cv_q) return cv_q < o.cv_q;
+    if (empty_q != o.empty_q) return empty_q < o.empty_q;
+    return nnz_q < o.nnz_q;
+  }
+};
+struct TuneResult {
+  int kernel, variant;
+  float ms[3];  // trial medians: row blocks, CSR5-style, column-sorted row blocks
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneResult> g_tune_cache;
+
+TuneKey tune_key(int val_type, const RowStats &s, int64_t nnz) {
+  auto q = [](double v, double per_octave) {
+    return static_cast<int>(std::lround(std::log2(std::max(v, 1e-3)) * per_octave));
+  };
+  return TuneKey{val_type, q(s.mean, 4), q(1.0 + s.cv, 4),
+                 static_cast<int>(std::lround(s.empty_frac * 10)), q(static_cast<double>(nnz), 1)};
+}
+
+}  // namespace
 // ---------------------------------------------------------------------------
 
 Plan::~Plan() {
   for (void *p : panel.buffers) (void)hipFree(p);
+  for (void *p : sorted.buffers) (void)hipFree(p);
   for (hipEvent_t e : tev) (void)hipEventDestroy(e);
   for (void *p : device_buffers) (void)hipFree(p);
   for (hipStream_t s : streams) (void)hipStreamDestroy(s);
@@ -109,6 +133,7 @@ void *Plan::dev_alloc(size_t bytes) {
   void *p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
   device_buffers.push_back(p);
+  buffer_bytes.push_back(workspace_bytes);
   workspace_bytes += static_cast<int64_t>(bytes);
   return p;
 }
@@ -394,7 +419,7 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
       if (st == SBLAS_OK)
         st = launch_csr5(val_type, sigma, tiles, nnz, d_c5_tile_nz, d_c5_flags, d_c5_rows_nz,
                          d_c5_colT, d_c5_valT, X, Y, alpha, beta, d_carry, s);
-    } else if (format == SBLAS_FMT_COO)
+    } else if (kernel == SBLAS_KERNEL_COO)
       st = launch_coo_tiles(val_type, tile_sub, m, nnz, tiles, a0, a1, val, X, Y, alpha, beta,
                             d_carry, s);
     else
@@ -409,8 +434,8 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     }
     return st;
   }
-  if (format == SBLAS_FMT_COO) {
-    set_error(SBLAS_ERR_UNSUPPORTED, "multi-vector products need a CSR or CSC plan");
+  if (kernel == SBLAS_KERNEL_COO) {
+    set_error(SBLAS_ERR_UNSUPPORTED, "multi-vector products need a CSR, CSC or row-pointer COO plan");
     return SBLAS_ERR_UNSUPPORTED;
   }
   if (variant == 3 && k == 1 && streams.empty()) {
@@ -438,6 +463,21 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     launches++;
     if (bgroups > 0 && merge_enabled) {
       st = launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, 1, Y, alpha, beta, s);
+      if (st != SBLAS_OK) return st;
+      launches++;
+    }
+    return SBLAS_OK;
+  }
+  if (variant == 4 && k == 1 && streams.empty()) {
+    hipEvent_t *te = timing_slot();
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
+    st = launch_csr_sorted(val_type, sorted, a0, X, Y, alpha, beta, s);
+    if (st != SBLAS_OK) return st;
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
+    launches++;
+    if (sgroups > 0 && merge_enabled) {
+      st = launch_merge(d_smrows, d_smptr, d_smslots, sgroups, sorted.partials, 1, Y, alpha, beta,
+                        s);
       if (st != SBLAS_OK) return st;
       launches++;
     }
@@ -497,7 +537,7 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
 }
 
 int Plan::setup_tiles() {
-  const int64_t items = (format == SBLAS_FMT_COO) ? nnz : static_cast<int64_t>(m) + nnz;
+  const int64_t items = (kernel == SBLAS_KERNEL_COO) ? nnz : static_cast<int64_t>(m) + nnz;
   tiles = static_cast<int>((items + kTileSize - 1) / kTileSize);
   tile_sub = opts.subwave > 0 ? opts.subwave : tile_subwave(stats.mean);
   if (tile_sub != 1 && tile_sub != 4 && tile_sub != 16 && tile_sub != 64) tile_sub = 16;
@@ -514,7 +554,7 @@ int Plan::setup_tiles() {
 // row start); per lane a word of row-start bits; the element arrays copied
 // in tile-transposed order (padded to whole tiles).
 int Plan::setup_csr5(const std::vector<int> &h_rp) {
-  const int sg = opts.sigma > 0 ? opts.sigma : (stats.mean < 8 ? 8 : 16);
+  const int sg = opts.sigma > 0 ? opts.sigma : autotune_sigma(nnz, m);
   if (sg != 8 && sg != 16 && sg != 32) {
     set_error(SBLAS_ERR_ARGS, "csr5: sigma must be 8, 16 or 32");
     return SBLAS_ERR_ARGS;
@@ -567,18 +607,164 @@ int Plan::setup_csr5(const std::vector<int> &h_rp) {
   return SBLAS_OK;
 }
 
+void Plan::free_buffers_from(size_t first) {
+  for (size_t i = first; i < device_buffers.size(); i++) (void)hipFree(device_buffers[i]);
+  device_buffers.resize(std::min(first, device_buffers.size()));
+  workspace_bytes = first < buffer_bytes.size() ? buffer_bytes[first] : workspace_bytes;
+  buffer_bytes.resize(std::min(first, buffer_bytes.size()));
+}
+
+// Column-sorted row blocks over the execution chunks (variant 4), with the
+// merge lists of the rows their windows cut.
+int Plan::setup_sorted(const int *h_rp) {
+  const std::vector<RowBlock> hb = build_row_blocks(exec, h_rp, kSortedCap);
+  int sub = opts.subwave > 0 ? opts.subwave : pow2_round(stats.mean / 4.0, 1, 64);
+  if (sub < 1 || sub > 64 || (sub & (sub - 1))) {
+    set_error(SBLAS_ERR_ARGS, "subwave must be a power of two in [1, 64]");
+    return SBLAS_ERR_ARGS;
+  }
+  int remap = opts.xcd_remap;
+  if (remap < 0) remap = env_int("SBLAS_XCD_REMAP", 1);
+  int st = sorted_build(val_type, n, a1, val, hb, sub, remap, sorted);
+  if (st != SBLAS_OK) return st;
+  workspace_bytes += sorted.bytes;
+  std::vector<ExecChunk> as_chunks;
+  as_chunks.reserve(hb.size());
+  for (const RowBlock &r : hb) as_chunks.push_back(ExecChunk{r.lo, r.hi, r.row_begin, r.row_end});
+  std::vector<int> mrows, mptr, mslots;
+  merge_lists(as_chunks, h_rp, mrows, mptr, mslots);
+  sgroups = static_cast<int>(mrows.size());
+  if ((st = upload(*this, &d_smrows, mrows)) != SBLAS_OK) return st;
+  if ((st = upload(*this, &d_smptr, mptr)) != SBLAS_OK) return st;
+  return upload(*this, &d_smslots, mslots);
+}
+
+// The column-sorted blocks' device arrays go (their merge lists are small
+// and stay).
+void Plan::drop_sorted() {
+  for (void *p : sorted.buffers) (void)hipFree(p);
+  workspace_bytes -= sorted.bytes;
+  sorted = SortedFormat{};
+  sgroups = 0;
+}
+
+// The CSR5-style format's buffers (allocated after `mark`) go.
+void Plan::drop_csr5(size_t mark) {
+  free_buffers_from(mark);
+  d_c5_tile_nz = d_c5_flags = nullptr;
+  d_c5_rows_nz = d_c5_empty = d_c5_colT = nullptr;
+  d_c5_valT = nullptr;
+  d_carry = nullptr;
+  tiles = 0;
+  sigma = 0;
+  c5_empty = 0;
+}
+
+// Times forms (kernel, variant) of the plan's own matrix with x = 0 and a
+// scratch y: launches cycle through the forms so a clock ramp affects all
+// alike, untimed for at least 20 ms, then five timed rounds; ms[i] is form
+// i's median.  The plan's kernel and variant are left at the last form.
+int Plan::trial(const std::vector<std::pair<int, int>> &forms, std::vector<float> &ms) {
+  const size_t F = forms.size();
+  ms.assign(F, 0.f);
+  double *tx = nullptr, *ty = nullptr;
+  hipStream_t ts = nullptr;
+  std::vector<hipEvent_t> ev(F + 1, nullptr);
+  int st = SBLAS_OK;
+  auto cleanup = [&]() {
+    if (ts) (void)hipStreamSynchronize(ts);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (ts) (void)hipStreamDestroy(ts);
+    if (tx) (void)hipFree(tx);
+    if (ty) (void)hipFree(ty);
+  };
+  if (hipMalloc(&tx, sizeof(double) * std::max(n, 1)) != hipSuccess ||
+      hipMalloc(&ty, sizeof(double) * std::max(m, 1)) != hipSuccess ||
+      hipStreamCreateWithFlags(&ts, hipStreamNonBlocking) != hipSuccess) {
+    cleanup();
+    set_error(SBLAS_ERR_ALLOC, "autotune: trial buffers");
+    return SBLAS_ERR_ALLOC;
+  }
+  for (hipEvent_t &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      cleanup();
+      set_error(SBLAS_ERR_HIP, "autotune: event creation failed");
+      return SBLAS_ERR_HIP;
+    }
+  if (hipMemsetAsync(tx, 0, sizeof(double) * std::max(n, 1), ts) != hipSuccess ||
+      hipMemsetAsync(ty, 0, sizeof(double) * std::max(m, 1), ts) != hipSuccess) {
+    cleanup();
+    set_error(SBLAS_ERR_HIP, "autotune: trial buffer set-up failed");
+    return SBLAS_ERR_HIP;
+  }
+  const bool was_timing = timing;
+  timing = false;
+  auto run_as = [&](const std::pair<int, int> &f) {
+    kernel = f.first;
+    variant = f.second;
+    return run_spmm(1, 1.0, tx, 0.0, ty, ts);
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  do {
+    for (int i = 0; i < 4 && st == SBLAS_OK; i++)
+      for (size_t f = 0; f < F && st == SBLAS_OK; f++) st = run_as(forms[f]);
+    if (st == SBLAS_OK && hipStreamSynchronize(ts) != hipSuccess) st = SBLAS_ERR_HIP;
+  } while (st == SBLAS_OK &&
+           std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(20));
+  std::vector<std::vector<float>> t(F);
+  for (int rep = 0; rep < 5 && st == SBLAS_OK; rep++) {
+    if (hipEventRecord(ev[0], ts) != hipSuccess) st = SBLAS_ERR_HIP;
+    for (size_t f = 0; f < F && st == SBLAS_OK; f++) {
+      st = run_as(forms[f]);
+      if (st == SBLAS_OK && hipEventRecord(ev[f + 1], ts) != hipSuccess) st = SBLAS_ERR_HIP;
+    }
+    if (st == SBLAS_OK && hipEventSynchronize(ev[F]) != hipSuccess) st = SBLAS_ERR_HIP;
+    for (size_t f = 0; f < F && st == SBLAS_OK; f++) {
+      float v = 0.f;
+      if (hipEventElapsedTime(&v, ev[f], ev[f + 1]) != hipSuccess) st = SBLAS_ERR_HIP;
+      t[f].push_back(v);
+    }
+  }
+  timing = was_timing;
+  cleanup();
+  if (st != SBLAS_OK) {
+    if (st == SBLAS_ERR_HIP) set_error(SBLAS_ERR_HIP, "autotune: trial launch failed");
+    return st;
+  }
+  for (size_t f = 0; f < F; f++) {
+    std::sort(t[f].begin(), t[f].end());
+    ms[f] = t[f][t[f].size() / 2];
+  }
+  return SBLAS_OK;
+}
+
 // Kernel choice and launch set-up for a CSR matrix whose row pointer is h_rp
 // (the caller's CSR, or the row-major form built from a CSC matrix).
+//
+// With kernel AUTO the autotuner decides from the matrix's structure and row
+// statistics: the column-panel form of kernel 1 where the column structure
+// pays for it (a structural test over the per-panel column ranges, measured
+// crossover); otherwise the row-block form of kernel 1 or the CSR5-style
+// kernel 3 (tile depth sigma from the reference's rule), whichever a timed
+// trial on this matrix finds faster.  Trials run for matrices of at least
+// 2^20 elements (smaller ones take row blocks) and are cached by row profile
+// (value type, mean, coefficient of variation, empty-row fraction, size),
+// so later plans of the same profile reuse the decision without a trial.
+// The merge path (kernel 2) is never faster than both on the measured
+// matrices and runs only when asked for.  SBLAS_AUTOTUNE=0 turns trials off.
 int Plan::finish_csr(const std::vector<int> &h_rp) {
   stats = row_stats(m, h_rp.data());
-  kernel = opts.kernel != SBLAS_KERNEL_AUTO ? opts.kernel : autotune_kernel(stats);
+  const bool auto_kernel = opts.kernel == SBLAS_KERNEL_AUTO;
+  kernel = auto_kernel ? SBLAS_KERNEL_VECTOR : opts.kernel;
+  tuned = auto_kernel ? 1 : 0;
   if (kernel < SBLAS_KERNEL_VECTOR || kernel > SBLAS_KERNEL_CSR5) {
     set_error(SBLAS_ERR_KERNEL, "CSR plans take kernel 1, 2 or 3");
     return SBLAS_ERR_KERNEL;
   }
   // kernel 1 form: row blocks staged in LDS unless the sub-wave form is asked
   // for; the sub-wave form also serves multi-RHS blocks and per-chunk streams
-  variant = (opts.variant == 1 || opts.streams > 1) ? 1 : 2;
+  variant = (opts.variant == 1 || opts.streams > 1) ? 1 : (opts.variant == 4 ? 4 : 2);
   if (opts.variant == 0) variant = env_int("SBLAS_VARIANT", variant);
   int st;
   if ((st = build_csr_chunks(h_rp.data())) != SBLAS_OK) return st;
@@ -593,7 +779,13 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     const int min_panels = 2 * std::max(cus, 1);
-    const int rows = opts.panel_rows > 0 ? opts.panel_rows : panel_rows_for(val_type, m, device);
+    // narrow values stage x in FP32 unless asked not to (SBLAS_NARROW_X
+    // overrides the auto setting)
+    int nxo = opts.narrow_x;
+    if (nxo == 0) nxo = env_int("SBLAS_NARROW_X", 0) > 0 ? env_int("SBLAS_NARROW_X", 0) : 2;
+    const bool narrow_x = val_type != SBLAS_VAL_F64 && nxo == 2;
+    const int rows =
+        opts.panel_rows > 0 ? opts.panel_rows : panel_rows_for(val_type, m, device, narrow_x);
     bool chosen = false;
     st = panel_build(val_type, rows, m, n, nnz, a0, a1, val, exec, h_rp.data(),
                      opts.variant == 3, min_panels, panel, &chosen, nullptr);
@@ -606,6 +798,20 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
       chosen = false;
     } else if (st != SBLAS_OK) {
       return st;
+    }
+    if (chosen && narrow_x) {
+      // FP32 copy of x, zero-padded to whole column panels (the pad is read
+      // by the x window loads, never used)
+      const int64_t padded = (static_cast<int64_t>(n) + kPanelCols - 1) / kPanelCols * kPanelCols;
+      void *p = nullptr;
+      if (hipMalloc(&p, sizeof(float) * std::max<int64_t>(padded, 4)) != hipSuccess) {
+        set_error(SBLAS_ERR_ALLOC, "plan: FP32 x copy allocation failed");
+        return SBLAS_ERR_ALLOC;
+      }
+      panel.buffers.push_back(p);
+      workspace_bytes += static_cast<int64_t>(sizeof(float)) * std::max<int64_t>(padded, 4);
+      SBLAS_HIP_TRY(hipMemset(p, 0, sizeof(float) * std::max<int64_t>(padded, 4)));
+      panel.x32 = static_cast<float *>(p);
     }
     if (chosen) {
       variant = 3;
@@ -622,18 +828,76 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
       return setup_stream(h_rp.data());  // row blocks serve multi-RHS products
     }
   }
-  if (variant == 2) return setup_stream(h_rp.data());
+  if (variant != 2 && variant != 4) return SBLAS_OK;
+  const bool want_sorted = variant == 4;
+  variant = 2;
+  if ((st = setup_stream(h_rp.data())) != SBLAS_OK) return st;  // also serves multi-RHS
+  if (want_sorted) {
+    variant = 4;
+    return setup_sorted(h_rp.data());
+  }
+  // kernel AUTO, no column panels: row blocks, column-sorted row blocks or
+  // the CSR5-style form
+  if (!auto_kernel || opts.variant != 0 || opts.streams > 1 || nnz < (int64_t{1} << 20))
+    return SBLAS_OK;
+  const TuneKey key = tune_key(val_type, stats, nnz);
+  TuneResult found{};
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      found = it->second;
+      cached = true;
+    }
+  }
+  if (cached) {
+    tuned = 3;
+    for (int i = 0; i < 3; i++) trial_ms[i] = found.ms[i];
+    if (found.kernel == SBLAS_KERNEL_CSR5) {
+      kernel = SBLAS_KERNEL_CSR5;
+      return setup_csr5(h_rp);
+    }
+    if (found.variant == 4) {
+      variant = 4;
+      return setup_sorted(h_rp.data());
+    }
+    return SBLAS_OK;
+  }
+  if (env_int("SBLAS_AUTOTUNE", 1) == 0) return SBLAS_OK;
+  // the sorted blocks' merge lists go into device_buffers before the mark,
+  // so dropping the CSR5 format (everything after the mark) leaves them
+  if ((st = setup_sorted(h_rp.data())) != SBLAS_OK) return st;
+  const size_t mark = device_buffers.size();
+  if ((st = setup_csr5(h_rp)) != SBLAS_OK) return st;
+  const std::vector<std::pair<int, int>> forms{
+      {SBLAS_KERNEL_VECTOR, 2}, {SBLAS_KERNEL_CSR5, 2}, {SBLAS_KERNEL_VECTOR, 4}};
+  std::vector<float> ms;
+  if ((st = trial(forms, ms)) != SBLAS_OK) return st;
+  tuned = 2;
+  size_t best = 0;
+  for (size_t f = 0; f < forms.size(); f++) {
+    trial_ms[f] = ms[f];
+    if (ms[f] < ms[best]) best = f;
+  }
+  kernel = forms[best].first;
+  variant = forms[best].second;
+  if (kernel != SBLAS_KERNEL_CSR5) drop_csr5(mark);
+  if (variant != 4) drop_sorted();
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = TuneResult{kernel, variant, {ms[0], ms[1], ms[2]}};
   return SBLAS_OK;
 }
 
 bool Plan::uses_tiles(int k) const {
-  return format == SBLAS_FMT_COO ||
+  return kernel == SBLAS_KERNEL_COO ||
          ((kernel == SBLAS_KERNEL_MERGE || kernel == SBLAS_KERNEL_CSR5) && k == 1);
 }
 
 bool Plan::has_merge(int k) const {
   if (uses_tiles(k)) return tiles > 0;
   if (variant == 3 && k == 1 && streams.empty()) return pgroups > 0;
+  if (variant == 4 && k == 1 && streams.empty()) return sgroups > 0;
   if (variant >= 2 && streams.empty()) return bgroups > 0;  // row blocks (any k)
   return ngroups > 0;
 }
@@ -643,6 +907,10 @@ int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) 
     return tiles > 0 ? launch_tile_fixup(carry_count(), d_carry, Y, alpha, beta, s) : SBLAS_OK;
   if (variant == 3 && k == 1 && streams.empty())
     return pgroups > 0 ? launch_merge(d_pmrows, d_pmptr, d_pmslots, pgroups, panel.partials, 1, Y,
+                                      alpha, beta, s)
+                       : SBLAS_OK;
+  if (variant == 4 && k == 1 && streams.empty())
+    return sgroups > 0 ? launch_merge(d_smrows, d_smptr, d_smslots, sgroups, sorted.partials, 1, Y,
                                       alpha, beta, s)
                        : SBLAS_OK;
   if (variant >= 2 && streams.empty())
@@ -729,7 +997,12 @@ static int plan_common(int format, int m, int n, int64_t nnz, const int *a0, con
     sblas::set_error(SBLAS_ERR_NO_DEVICE, "plan: no HIP device");
     return SBLAS_ERR_NO_DEVICE;
   }
-  int C = p.opts.chunks > 0 ? p.opts.chunks : sblas::env_int("SBLAS_CHUNKS", 1);
+  // chunks = 0: SBLAS_CHUNKS when set, else the autotuner's rule -- one chunk
+  // for a single launch (its XCD-contiguous block order already gives each
+  // XCD a contiguous eighth of the rows, the role the reference's per-GPU
+  // chunks play), one chunk per stream when chunks run over several streams
+  int C = p.opts.chunks > 0 ? p.opts.chunks : sblas::env_int("SBLAS_CHUNKS", 0);
+  if (C <= 0) C = std::max(p.opts.streams, 1);
   p.chunks = std::max(1, C);
   *out = h;
   return SBLAS_OK;
@@ -795,12 +1068,37 @@ int sblas_plan_create_coo(int m, int n, int64_t nnz, const int *d_row_idx, const
                      "plan: COO row indices must be sorted in [0, m) and columns in [0, n)");
     return fail(SBLAS_ERR_ARGS);
   }
-  p.kernel = SBLAS_KERNEL_COO;
-  // descriptors of the COO rules need the row indices on the host:
-  // sblas_partition_v1_numa_coo / sblas_partition_baseline_coo give them
-  p.desc_status = SBLAS_ERR_UNSUPPORTED;
-  p.stats.mean = m > 0 ? static_cast<double>(nnz) / m : 0.0;
-  if ((st = p.setup_tiles()) != SBLAS_OK) return fail(st);
+  if (p.opts.kernel == SBLAS_KERNEL_COO) {
+    // the segmented reduction over the row-sorted COO arrays as given
+    p.kernel = SBLAS_KERNEL_COO;
+    // descriptors of the COO rules need the row indices on the host:
+    // sblas_partition_v1_numa_coo / sblas_partition_baseline_coo give them
+    p.desc_status = SBLAS_ERR_UNSUPPORTED;
+    p.stats.mean = m > 0 ? static_cast<double>(nnz) / m : 0.0;
+    if ((st = p.setup_tiles()) != SBLAS_OK) return fail(st);
+    return SBLAS_OK;
+  }
+  // the reference's COO path (coo2csr, src/spmv_helper.cu:478-491, then the
+  // CSR kernel, src/dspmv_mgpu_v1_numa_coo.cu:460-476): row pointers built on
+  // the device from the sorted row indices, then the CSR forms over the
+  // caller's column and value arrays (the row indices are not read again)
+  p.d_csr_rp = static_cast<int *>(p.dev_alloc(sizeof(int) * (static_cast<size_t>(m) + 1)));
+  if (!p.d_csr_rp) {
+    sblas::set_error(SBLAS_ERR_ALLOC, "plan: COO row pointer allocation failed");
+    return fail(SBLAS_ERR_ALLOC);
+  }
+  if ((st = sblas::ptr_from_sorted_device(m, nnz, d_row_idx, p.d_csr_rp, nullptr)) != SBLAS_OK)
+    return fail(st);
+  std::vector<int> h_rp(static_cast<size_t>(m) + 1);
+  if (hipMemcpy(h_rp.data(), p.d_csr_rp, sizeof(int) * h_rp.size(), hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    sblas::set_error(SBLAS_ERR_HIP, "plan: row pointer download failed");
+    return fail(SBLAS_ERR_HIP);
+  }
+  p.a0 = p.d_csr_rp;
+  if ((st = p.finish_csr(h_rp)) != SBLAS_OK) return fail(st);
+  p.desc_status = SBLAS_ERR_UNSUPPORTED;  // the COO rules: see above
+  p.descs.clear();
   return SBLAS_OK;
 }
 
@@ -926,6 +1224,8 @@ int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info) {
   info->items = p.sl.items;
   info->panel_rows = p.variant == 3 ? p.panel.rows : 0;
   info->panel_slices = p.variant == 3 ? p.panel.nslices : 0;
+  info->tuned = p.tuned;
+  for (int i = 0; i < 3; i++) info->trial_ms[i] = p.trial_ms[i];
   if (p.variant == 2) info->subwave = p.sl.subwave;
   return SBLAS_OK;
 }
@@ -957,6 +1257,10 @@ int sblas_internal_row_blocks(int m, const int *rp, int64_t nnz, int C, int part
   }
   return n;
 }
+
+// Host-only view of the CSR5-style tile depth rule (not in sblas.h; used by
+// the CPU tests).
+int sblas_internal_autotune_sigma(int64_t nnz, int m) { return sblas::autotune_sigma(nnz, m); }
 
 int sblas_plan_set_timing(sblas_plan plan, int on) {
   if (!plan) return SBLAS_ERR_ARGS;

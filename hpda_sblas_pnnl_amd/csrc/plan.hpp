@@ -13,13 +13,11 @@ struct RowStats {
 };
 
 RowStats row_stats(int m, const int *rp);
-int autotune_kernel(const RowStats &s);
 int autotune_subwave(const RowStats &s);
-int autotune_sigma(const RowStats &s);
+int autotune_sigma(int64_t nnz, int m);
 std::vector<RowBlock> build_row_blocks(const std::vector<ExecChunk> &ex, const int *rp,
                                        int cap);
 const char *last_message();
-void clear_error();
 // Vector-kernel (sub-wave) launch shape over execution chunks; dev receives
 // one DevChunk per chunk.
 VectorLaunch vector_shape(const std::vector<ExecChunk> &ex, double mean, int subwave,
@@ -82,6 +80,11 @@ struct Plan {
   int *d_pmrows = nullptr, *d_pmptr = nullptr, *d_pmslots = nullptr;
   int pgroups = 0;
 
+  // column-sorted row blocks of kernel 1 (variant 4); rows cut by block edges
+  SortedFormat sorted;
+  int *d_smrows = nullptr, *d_smptr = nullptr, *d_smslots = nullptr;
+  int sgroups = 0;
+
   // CSR5-style format (kernel 3)
   uint32_t *d_c5_tile_nz = nullptr, *d_c5_flags = nullptr;
   int *d_c5_rows_nz = nullptr, *d_c5_empty = nullptr;
@@ -96,7 +99,12 @@ struct Plan {
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
   std::vector<void *> device_buffers;
+  std::vector<int64_t> buffer_bytes;  // workspace_bytes before each buffer
   int64_t workspace_bytes = 0;
+  // how the kernel was chosen: 0 asked for, 1 rule, 2 timed trial, 3 cached
+  // trial; the trial's median times (row blocks, CSR5-style) in ms
+  int tuned = 0;
+  float trial_ms[3] = {0.f, 0.f, 0.f};  // row blocks, CSR5-style, column-sorted row blocks
   int launches = 0;
 
   // kernel timing (measurement only)
@@ -119,6 +127,13 @@ struct Plan {
   int setup_tiles();
   int setup_csr5(const std::vector<int> &h_rp);
   int finish_csr(const std::vector<int> &h_rp);
+  void free_buffers_from(size_t first);
+  int setup_sorted(const int *h_rp);
+  void drop_sorted();
+  void drop_csr5(size_t mark);
+  // times the given (kernel, variant) forms on the plan's own matrix; ms[i]
+  // receives the median of form i
+  int trial(const std::vector<std::pair<int, int>> &forms, std::vector<float> &ms);
   int run_merge(int k, double alpha, double beta, double *Y, hipStream_t s);
   hipEvent_t *timing_slot();
   int run_spmv(double alpha, const double *x, double beta, double *y, hipStream_t s);

@@ -43,14 +43,14 @@ __global__ void k_fill_u32(int64_t count, uint32_t *__restrict__ a) {
 
 namespace {
 
-template <typename VT, int RPT, typename OT>
+template <typename VT, int RPT, typename OT, bool NX>
 __global__ __launch_bounds__(256) void k_panel_spmv(
     int n, int R, int npanels, int xcd_remap, const PanelInfo *__restrict__ info,
     const int *__restrict__ slice_base, const int64_t *__restrict__ slice_off,
     const OT *__restrict__ srp, const VT *__restrict__ pval,
     const uint16_t *__restrict__ pcol, const double *__restrict__ x, double *__restrict__ y,
-    double alpha, double beta, double *__restrict__ partials) {
-  __shared__ double lds[panel::lds_doubles<OT, RPT>()];
+    double alpha, double beta, double *__restrict__ partials, const float *__restrict__ x32) {
+  __shared__ double lds[panel::lds_doubles<OT, RPT, NX>()];
   int p = blockIdx.x;
   if (xcd_remap) {  // consecutive row panels on one XCD share their x window in its L2
     const int per = gridDim.x >> 3;
@@ -58,8 +58,14 @@ __global__ __launch_bounds__(256) void k_panel_spmv(
   }
   if (p >= npanels) return;  // uniform per workgroup
   const PanelInfo pi = info[p];
-  panel::row_panel<VT, RPT, OT>(p, n, R, pi, slice_base, slice_off, srp, pval, pcol, x, y, alpha,
-                                beta, partials + 2 * p, lds);
+  panel::row_panel<VT, RPT, OT, NX>(p, n, R, pi, slice_base, slice_off, srp, pval, pcol, x, y,
+                                    alpha, beta, partials + 2 * p, lds, x32);
+}
+
+// x32[i] = x[i] rounded to FP32 (the narrow-x column-panel form's x copy)
+__global__ void k_x_to_f32(int n, const double *__restrict__ x, float *__restrict__ x32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x32[i] = static_cast<float>(x[i]);
 }
 
 }  // namespace
@@ -432,16 +438,16 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
 
 namespace {
 
-template <typename VT, typename OT>
+template <typename VT, typename OT, bool NX>
 int panel_launch_typed(const PanelFormat &f, const OT *srp, int m, int n, int xcd_remap,
                        const double *x, double *y, double alpha, double beta, hipStream_t s) {
   const int grid = xcd_remap ? ((f.npanels + 7) / 8) * 8 : f.npanels;
 #define SBLAS_PANEL_CASE(RPT)                                                                 \
   case RPT:                                                                                   \
-    hipLaunchKernelGGL((k_panel_spmv<VT, RPT, OT>), dim3(grid), dim3(256), 0, s, n, f.rows,    \
-                       f.npanels, xcd_remap, f.info, f.slice_base, f.slice_off, srp,          \
+    hipLaunchKernelGGL((k_panel_spmv<VT, RPT, OT, NX>), dim3(grid), dim3(256), 0, s, n,        \
+                       f.rows, f.npanels, xcd_remap, f.info, f.slice_base, f.slice_off, srp,  \
                        static_cast<const VT *>(f.pval), f.pcol, x, y, alpha, beta,           \
-                       f.partials);                                                           \
+                       f.partials, f.x32);                                                    \
     break;
   switch (f.rpt) {
     SBLAS_PANEL_CASE(1)
@@ -458,11 +464,25 @@ int panel_launch_typed(const PanelFormat &f, const OT *srp, int m, int n, int xc
   return SBLAS_OK;
 }
 
+template <typename VT, bool NX>
+int panel_launch_nx(const PanelFormat &f, int m, int n, int xcd_remap, const double *x, double *y,
+                    double alpha, double beta, hipStream_t s) {
+  return f.srp ? panel_launch_typed<VT, uint16_t, NX>(f, f.srp, m, n, xcd_remap, x, y, alpha,
+                                                      beta, s)
+               : panel_launch_typed<VT, uint32_t, NX>(f, f.srp32, m, n, xcd_remap, x, y, alpha,
+                                                      beta, s);
+}
+
 template <typename VT>
 int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const double *x, double *y,
                     double alpha, double beta, hipStream_t s) {
-  return f.srp ? panel_launch_typed<VT>(f, f.srp, m, n, xcd_remap, x, y, alpha, beta, s)
-               : panel_launch_typed<VT>(f, f.srp32, m, n, xcd_remap, x, y, alpha, beta, s);
+  if (f.x32) {
+    // the FP32 copy of x first, on the same stream
+    hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, f.x32);
+    SBLAS_HIP_TRY(hipGetLastError());
+    return panel_launch_nx<VT, true>(f, m, n, xcd_remap, x, y, alpha, beta, s);
+  }
+  return panel_launch_nx<VT, false>(f, m, n, xcd_remap, x, y, alpha, beta, s);
 }
 
 }  // namespace
@@ -472,14 +492,18 @@ int panel_rpt(int rows) {
   return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : 8;
 }
 
-int panel_rows_for(int val_type, int m, int device) {
+int panel_rows_for(int val_type, int m, int device, bool narrow_x) {
   int cus = 256, per_cu = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  const void *fn = val_type == SBLAS_VAL_F64
-                       ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t>)
-                   : val_type == SBLAS_VAL_F32
-                       ? reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t>)
-                       : reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t>);
+  const void *fn =
+      val_type == SBLAS_VAL_F64
+          ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t, false>)
+      : val_type == SBLAS_VAL_F32
+          ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t, true>)
+                      : reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t, false>))
+          : (narrow_x
+                 ? reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t, true>)
+                 : reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t, false>));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 2;

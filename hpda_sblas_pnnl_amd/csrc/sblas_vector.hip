@@ -146,14 +146,16 @@ __global__ __launch_bounds__(BLOCK) void k_csr_vector_mrhs(
   }
 }
 
-// LDS-staged row blocks.  Phase 1 streams the block's element window with
-// consecutive threads on consecutive elements (every load instruction of a
-// wave covers whole cache lines; ITEMS loads of values and of columns are
-// issued back to back before any x gather), and stores the FP64 products in
-// LDS.  Phase 2 sums each row's products with SUBW lanes per row; phase 3
-// writes the block's y entries with consecutive threads on consecutive rows.
-// A block that holds a piece of one long row sums in registers and reduces
-// across the whole workgroup instead.
+// LDS-staged row blocks.  Phase 1 issues every global load the block needs
+// that does not depend on another: ITEMS values and columns per thread
+// (consecutive threads on consecutive elements, so each load instruction of a
+// wave covers whole cache lines), the block's row pointers and, with
+// beta != 0, its y entries; then the x gathers.  Row pointers go to LDS and y
+// stays in registers, so the row sums (phase 2, SUBW lanes per row, products
+// read from LDS) and the y writes (phase 3, consecutive threads on consecutive
+// rows) wait on no further memory round trip.  A block that holds a piece of
+// one long row sums in registers and reduces across the whole workgroup
+// instead.
 template <typename VT, int ITEMS, int SUBW, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_csr_stream(
     const RowBlock *__restrict__ blocks, int nblocks, int xcd_remap,
@@ -161,20 +163,38 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
     const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
     double *__restrict__ partials) {
   constexpr int CAP = BLOCK * ITEMS;
+  constexpr int RPT = (kStreamRowsMax + BLOCK) / BLOCK;  // row pointers per thread
+  constexpr int YPT = kStreamRowsMax / BLOCK;            // rows per thread in phase 3
   __shared__ double prod[CAP];
   __shared__ double ysh[kStreamRowsMax];
+  __shared__ int rps[kStreamRowsMax + 1];
   __shared__ double wsum[BLOCK / 64];
   const int b = logical_block(nblocks, xcd_remap);
   if (b < 0) return;
   const RowBlock rb = blocks[b];
   const int lo = rb.lo, hi = rb.hi;
   const int t = threadIdx.x;
+  const int r0 = rb.row_begin, nrows = rb.row_end - rb.row_begin;
 
-  // phase 1: products of the window, ITEMS per thread, strided by BLOCK.
-  // Lanes past the window re-read its last element (same cache line) and
-  // their product is dropped by a select, so no load sits under a branch and
-  // all ITEMS value, column and x loads are in flight together.
+  // phase 1: all independent loads first.  Lanes past the window re-read its
+  // last element (same cache line) and their product is dropped by a select;
+  // row pointer and y addresses are clamped into the block's rows, so no load
+  // sits under a branch.
   double p[ITEMS];
+  // the first row's bounds, the same for every thread (the long-row test
+  // below must be uniform over the workgroup)
+  const int rp_first = rp[r0], rp_second = rp[r0 + 1];
+  int rpv[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) rpv[k] = rp[r0 + min(t + k * BLOCK, nrows)];
+  double yv[YPT];
+  if (beta != 0.0) {
+#pragma unroll
+    for (int k = 0; k < YPT; k++) yv[k] = y[r0 + min(t + k * BLOCK, nrows - 1)];
+  } else {
+#pragma unroll
+    for (int k = 0; k < YPT; k++) yv[k] = 0.0;
+  }
   if (hi > lo) {
     double v[ITEMS];
     int c[ITEMS];
@@ -194,8 +214,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
     for (int k = 0; k < ITEMS; k++) p[k] = 0.0;
   }
 
-  const int nrows = rb.row_end - rb.row_begin;
-  if (nrows == 1 && (rp[rb.row_begin] < lo || rp[rb.row_begin + 1] > hi)) {
+  if (nrows == 1 && (rp_first < lo || rp_second > hi)) {
     // piece of a long row: register sum, wave reduction, then across waves
     double s = 0.0;
 #pragma unroll
@@ -216,14 +235,16 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
     const int e = t + k * BLOCK;
     if (e < hi - lo) prod[e] = p[k];
   }
+#pragma unroll
+  for (int k = 0; k < RPT; k++)
+    if (t + k * BLOCK <= nrows) rps[t + k * BLOCK] = rpv[k];
   __syncthreads();
 
   // phase 2: one sub-wave of SUBW lanes per row
   constexpr int SUBS = BLOCK / SUBW;
   const int sub = t / SUBW, lane = t % SUBW;
   for (int i = sub; i < nrows; i += SUBS) {
-    const int r = rb.row_begin + i;
-    const int s0 = max(rp[r], lo) - lo, e0 = min(rp[r + 1], hi) - lo;
+    const int s0 = max(rps[i], lo) - lo, e0 = min(rps[i + 1], hi) - lo;
     double s = 0.0;
     for (int j = s0 + lane; j < e0; j += SUBW) s += prod[j];
     if constexpr (SUBW > 1) s = subwave_sum<SUBW>(s);
@@ -232,14 +253,17 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
   __syncthreads();
 
   // phase 3: y for whole rows, partials for rows cut by the window
-  for (int i = t; i < nrows; i += BLOCK) {
-    const int r = rb.row_begin + i;
-    const double s = ysh[i];
-    const bool edge = (i == 0 || i == nrows - 1);
-    if (edge && (rp[r] < lo || rp[r + 1] > hi)) {
-      partials[2 * b + (i == 0 ? 0 : 1)] = s;
-    } else {
-      y[r] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[r];
+#pragma unroll
+  for (int k = 0; k < YPT; k++) {
+    const int i = t + k * BLOCK;
+    if (i < nrows) {
+      const double s = ysh[i];
+      const bool edge = (i == 0 || i == nrows - 1);
+      if (edge && (rps[i] < lo || rps[i + 1] > hi)) {
+        partials[2 * b + (i == 0 ? 0 : 1)] = s;
+      } else {
+        y[r0 + i] = (beta == 0.0) ? alpha * s : alpha * s + beta * yv[k];
+      }
     }
   }
 }

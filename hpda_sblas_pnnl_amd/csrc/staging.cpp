@@ -10,6 +10,8 @@
 // consume them can start while later pieces are still on their way.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -22,6 +24,17 @@ namespace {
 constexpr size_t kSlotBytes = size_t{64} << 20;  // one piece
 constexpr int kSlots = 3;
 constexpr unsigned kThreadsMax = 16;               // host copy threads (the caller among them)
+constexpr int kMaxDevices = 64;
+
+// The process-wide rings, one per device, each behind its own mutex.  Rings
+// are never destroyed: their pinned buffers live until the process ends, and
+// no HIP call runs from a static destructor.
+struct RingHolder {
+  std::mutex mu;
+  PinnedRing *ring = nullptr;
+  bool alloc_failed = false;  // pinned allocation failed once: copy directly from then on
+};
+RingHolder g_rings[kMaxDevices];
 
 }  // namespace
 
@@ -62,6 +75,12 @@ int PinnedRing::init() {
   return SBLAS_OK;
 }
 
+int PinnedRing::wait_all() {
+  for (size_t k = 0; k < slots.size(); k++)
+    if (used[k]) SBLAS_HIP_TRY(hipEventSynchronize(events[k]));
+  return SBLAS_OK;
+}
+
 int PinnedRing::to_device(void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (bytes == 0) return SBLAS_OK;
   int st = init();
@@ -79,6 +98,43 @@ int PinnedRing::to_device(void *dst, const void *src, size_t bytes, hipStream_t 
     SBLAS_HIP_TRY(hipEventRecord(events[k], s));
     used[k] = true;
   }
+  return SBLAS_OK;
+}
+
+StagingLease::StagingLease(size_t total_bytes) {
+  if (total_bytes < kStageMinBytes) return;
+  if (hipGetDevice(&device_) != hipSuccess || device_ < 0 || device_ >= kMaxDevices) return;
+  RingHolder &h = g_rings[device_];
+  if (!h.mu.try_lock()) return;  // another call is staging on this device
+  if (h.alloc_failed) {
+    h.mu.unlock();
+    return;
+  }
+  if (!h.ring) {
+    PinnedRing *r = new (std::nothrow) PinnedRing;
+    if (!r || r->init() != SBLAS_OK) {
+      // pinned memory unavailable: the calls copy from pageable memory
+      delete r;
+      h.alloc_failed = true;
+      h.mu.unlock();
+      clear_error();
+      return;
+    }
+    h.ring = r;
+  }
+  ring_ = h.ring;
+}
+
+StagingLease::~StagingLease() {
+  if (!ring_) return;
+  (void)ring_->wait_all();
+  g_rings[device_].mu.unlock();
+}
+
+int StagingLease::to_device(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return SBLAS_OK;
+  if (ring_) return ring_->to_device(dst, src, bytes, s);
+  SBLAS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
   return SBLAS_OK;
 }
 

@@ -127,7 +127,9 @@ int stream_spmv_csr(int m, int n, int64_t nnz, const int *h_rp, const int *h_col
     S.st.push_back(x);
     S.ev.push_back(e);
   }
-  PinnedRing ring;  // destroyed (after its transfers) before the streams
+  // matrix copies through the process-wide staging ring; the lease waits for
+  // its transfers when it ends, before the streams are destroyed
+  StagingLease ring(static_cast<size_t>(nnz) * (sizeof(int) + sizeof(double)));
   hipEvent_t start = nullptr;
   if (hipEventCreateWithFlags(&start, hipEventDisableTiming) != hipSuccess) {
     set_error(SBLAS_ERR_HIP, "stream spmv: event creation failed");

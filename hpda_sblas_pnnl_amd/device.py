@@ -14,7 +14,7 @@ from ._lib import Opts, PlanInfo, check, lib
 
 _VAL_TYPES = {"f64": _lib.VAL_F64, "f32": _lib.VAL_F32, "bf16": _lib.VAL_BF16}
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "vector": _lib.KERNEL_VECTOR, "merge": _lib.KERNEL_MERGE,
-            "csr5": _lib.KERNEL_CSR5, 0: 0, 1: 1, 2: 2, 3: 3}
+            "csr5": _lib.KERNEL_CSR5, "coo": 4, 0: 0, 1: 1, 2: 2, 3: 3, 4: 4}
 _PARTS = {"v1": _lib.PART_V1, "v1_numa": _lib.PART_V1_NUMA, "v2": _lib.PART_V2,
           "rowblock": _lib.PART_ROWBLOCK}
 
@@ -36,7 +36,7 @@ def val_type_of(t) -> int:
 
 
 def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=0, sigma=0,
-              xcd_remap=-1, variant=0, items=0, panel_rows=0) -> Opts:
+              xcd_remap=-1, variant=0, items=0, panel_rows=0, narrow_x=0) -> Opts:
     o = Opts()
     lib().sblas_opts_default(ctypes.byref(o))
     o.kernel = _KERNELS[kernel]
@@ -50,6 +50,7 @@ def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=
     o.variant = variant
     o.items = items
     o.panel_rows = panel_rows
+    o.narrow_x = narrow_x
     return o
 
 

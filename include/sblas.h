@@ -187,9 +187,19 @@ enum {
   SBLAS_PART_ROWBLOCK = 3   /* src/dspmv_mgpu_baseline.cu:64-70             */
 };
 
+/* kernel AUTO: column panels (kernel 1) where the matrix's column structure
+ * pays for them; otherwise row blocks or column-sorted row blocks (kernel 1)
+ * or the CSR5-style kernel 3, whichever a timed trial on the matrix finds
+ * fastest (matrices of at least 2^20 elements; smaller ones take row
+ * blocks).  Trials are cached by row
+ * profile; SBLAS_AUTOTUNE=0 turns them off.  sigma 0: the reference's rule
+ * (include/anonymouslib_cuda.h:294-318) snapped to 8, 16 or 32.
+ * chunks 0: SBLAS_CHUNKS when set, else one chunk per stream (one chunk for
+ * a single launch, whose block order already gives each XCD a contiguous
+ * range of rows). */
 typedef struct sblas_opts {
   int kernel;      /* SBLAS_KERNEL_*; AUTO lets the autotuner choose        */
-  int chunks;      /* chunk count C (0 = autotuner)                         */
+  int chunks;      /* chunk count C (0: see above)                          */
   int part_mode;   /* SBLAS_PART_*                                          */
   int nb;          /* v2 task size in nnz (part_mode V2)                    */
   int streams;     /* >1: one launch per chunk over this many streams       */
@@ -197,11 +207,18 @@ typedef struct sblas_opts {
   int sigma;       /* CSR5 nnz per lane per tile (0 = autotuned)            */
   int xcd_remap;   /* 1: contiguous row ranges per XCD; 0: off; -1: auto   */
   int variant;     /* kernel 1 form: 0 auto, 1 sub-wave per row, 2 row blocks */
-                   /* staged in LDS, 3 column panels with x staged in LDS    */
+                   /* staged in LDS, 3 column panels with x staged in LDS,   */
+                   /* 4 row blocks stored sorted by column (x read in column */
+                   /* order, products put back in element order in LDS)      */
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
   int panel_rows;  /* column-panel form: rows per row panel (1..2048, or   */
                    /* 1..1536 in a batch; 0 auto)                           */
-  int reserved[5];
+  int narrow_x;    /* FP32 / BF16 values in the column-panel form: 0 auto   */
+                   /* (= on), 1 off: x staged in FP64, each row summed in   */
+                   /* the host loop's order; 2 on: x staged from an FP32    */
+                   /* copy, FP32 products summed in FP64 (error within a    */
+                   /* few FP32 roundings per product)                        */
+  int reserved[4];
 } sblas_opts;
 
 /* Fills opts with defaults (auto kernel, C=1, v1 partition, fused launch). */
@@ -242,7 +259,12 @@ typedef struct sblas_plan_info {
   int items;         /* elements per thread of the row-block form           */
   int panel_rows;    /* column-panel form: rows per row panel               */
   int panel_slices;  /* column-panel form: (row panel, column panel) slices */
-  int reserved[2];
+  int tuned;         /* how the kernel was chosen: 0 asked for (opts.kernel */
+                     /* not AUTO), 1 by rule, 2 by a timed trial on this     */
+                     /* matrix, 3 by a cached trial of the same row profile  */
+  float trial_ms[3]; /* trial medians in ms: row blocks (kernel 1 variant   */
+                     /* 2), CSR5-style (kernel 3), column-sorted row blocks  */
+                     /* (kernel 1 variant 4); 0 without a trial              */
 } sblas_plan_info;
 int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info);
 /* Chunk descriptors of the plan (reference rule of opts.part_mode). */
@@ -288,6 +310,10 @@ typedef struct sblas_batch_choice {
   int variant;     /* 2 row blocks, 3 column panels (x staged in LDS)        */
   double mean_row, cv_row;
   int64_t max_row;
+  int panel_rows;  /* column panels: rows per row panel (shorter than the    */
+                   /* default when slices would overflow 16-bit offsets);    */
+                   /* 0 for row blocks                                       */
+  int reserved_;
 } sblas_batch_choice;
 
 typedef struct sblas_batch_s *sblas_batch;

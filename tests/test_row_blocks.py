@@ -69,3 +69,21 @@ def test_long_rows_and_empty():
     rp0 = np.zeros(7, np.int32)
     B = _check(6, rp0, 4, 0, 8)
     assert len(B) == 1 and tuple(B[0]) == (0, 6, 0, 0)
+
+
+def test_csr5_sigma_rule():
+    """Tile depth of the CSR5-style form with sigma = 0: the reference's rule
+    on integer nnz / m (include/anonymouslib_cuda.h:294-318: 4 up to 4, nnz/m
+    up to 32, 32 up to 256, 6 beyond) snapped to the depths the kernel is
+    built for, nearest in log2 (midpoints 8*sqrt(2) = 11.3, 16*sqrt(2) = 22.6)."""
+    from hpda_sblas_pnnl_amd import _lib
+    f = _lib.lib().sblas_internal_autotune_sigma
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int64, ctypes.c_int]
+    m = 1000
+    cases = {1: 8, 4: 8, 5: 8, 11: 8, 12: 16, 16: 16, 22: 16, 23: 32, 32: 32, 33: 32, 140: 32,
+             256: 32, 257: 8, 2000: 8}
+    for per_row, want in cases.items():
+        assert f(per_row * m, m) == want, per_row
+        assert f(per_row * m + m - 1, m) == want, per_row  # integer division, as the reference
+    assert f(0, 0) == 8
