@@ -120,6 +120,7 @@ SIGNATURES = {
     "sblas_plan_create_csc": (_I, [_I, _I, _I64, _P, _P, _P, _I, ctypes.POINTER(Opts),
                                    ctypes.POINTER(_P)]),
     "sblas_plan_spmv": (_I, [_P, _D, _P, _D, _P, _P]),
+    "sblas_spmv_csr_noalloc": (_I, [_I, _I, _I64, _P, _P, _P, _I, _D, _P, _D, _P, _P]),
     "sblas_plan_spmm": (_I, [_P, _I, _D, _P, _D, _P, _P]),
     "sblas_plan_get_info": (_I, [_P, ctypes.POINTER(PlanInfo)]),
     "sblas_plan_get_chunks": (_I, [_P, _P, _I]),

@@ -11,6 +11,10 @@
 // in tile order, so every row's sum has a fixed order: no atomics on y.
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
+#include <mutex>
+#include <string>
+
 #include "internal.hpp"
 
 namespace sblas {
@@ -397,3 +401,84 @@ int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha
 }
 
 }  // namespace sblas
+
+// ---------------------------------------------------------------------------
+// Merge-path SpMV that allocates nothing: the same two launches as a kernel-2
+// plan (k_csr_merge_path, then k_tile_fixup), with the per-tile carries in a
+// file-scope device array instead of a plan's workspace.  Every tile writes
+// its carry on every call before the fix-up reads it, so nothing needs
+// clearing between calls.  One array per GPU, shared by every call on that
+// GPU: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace sblas {
+namespace {
+
+constexpr int kNoAllocTiles = 262144;  // 8 MiB of carries: m + nnz <= 2^29 merge items
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) TileCarry scratch[kNoAllocTiles];
+
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (each GPU holds its own copy of the array).
+struct ScratchAddress {
+  std::once_flag once;
+  TileCarry *ptr = nullptr;
+  hipError_t status = hipSuccess;
+};
+ScratchAddress g_scratch[kNoAllocDevices];
+
+int scratch_for_current_device(TileCarry **out) {
+  int dev = -1;
+  SBLAS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error(SBLAS_ERR_UNSUPPORTED, "spmv noalloc: device index beyond the scratch table");
+    return SBLAS_ERR_UNSUPPORTED;
+  }
+  ScratchAddress &a = g_scratch[dev];
+  std::call_once(a.once, [&a] {
+    void *p = nullptr;
+    a.status = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    a.ptr = static_cast<TileCarry *>(p);
+  });
+  if (a.status != hipSuccess || !a.ptr) {
+    set_error(SBLAS_ERR_HIP, std::string("spmv noalloc: hipGetSymbolAddress(scratch): ") +
+                                 hipGetErrorString(a.status));
+    return SBLAS_ERR_HIP;
+  }
+  *out = a.ptr;
+  return SBLAS_OK;
+}
+
+}  // namespace
+}  // namespace sblas
+
+extern "C" int sblas_spmv_csr_noalloc(int m, int n, int64_t nnz, const int *d_row_ptr,
+                                      const int *d_col, const void *d_val, int val_type,
+                                      double alpha, const double *d_x, double beta, double *d_y,
+                                      void *stream) {
+  using namespace sblas;
+  clear_error();
+  if (m < 0 || n < 0 || nnz < 0 || nnz > INT_MAX - 4096 || (m > 0 && (!d_row_ptr || !d_y)) ||
+      (nnz > 0 && (!d_col || !d_val || !d_x)) || val_type < SBLAS_VAL_F64 ||
+      val_type > SBLAS_VAL_BF16) {
+    set_error(SBLAS_ERR_ARGS, "spmv noalloc: invalid sizes, pointers or value type");
+    return SBLAS_ERR_ARGS;
+  }
+  if (m == 0) return SBLAS_OK;
+  const int64_t items = static_cast<int64_t>(m) + nnz;
+  const int64_t tiles = (items + kTileSize - 1) / kTileSize;
+  if (tiles > kNoAllocTiles) {
+    set_error(SBLAS_ERR_ARGS, "spmv noalloc: m + nnz exceeds 2^29 (the scratch holds 262144 "
+                              "tile carries of 2048 merge items)");
+    return SBLAS_ERR_ARGS;
+  }
+  TileCarry *carry = nullptr;
+  int st = scratch_for_current_device(&carry);
+  if (st != SBLAS_OK) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int sub = tile_subwave(static_cast<double>(nnz) / m);
+  st = launch_csr_merge_path(val_type, sub, m, nnz, static_cast<int>(tiles), d_row_ptr, d_col,
+                             d_val, d_x, d_y, alpha, beta, carry, s);
+  if (st != SBLAS_OK) return st;
+  return launch_tile_fixup(static_cast<int>(tiles), carry, d_y, alpha, beta, s);
+}

@@ -221,6 +221,23 @@ typedef struct sblas_opts {
   int reserved[4];
 } sblas_opts;
 
+/* y = alpha*A*x + beta*y for a CSR matrix on the device, merge path (kernel
+ * 2), WITHOUT ANY ALLOCATION: no plan, no hipMalloc, no workspace argument.
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * allocation).  The per-tile carries live in the library's own device array
+ * of 262144 carries (8 MiB per GPU, held until the process ends), so
+ * m + nnz must not exceed 2^29 (SBLAS_ERR_ARGS otherwise, before any device
+ * work).  The same kernels and launch shape as a kernel-2 plan, so results
+ * equal that plan's bit for bit.  Inputs are not validated (a plan created by
+ * sblas_plan_create_csr validates them): row pointers must be non-decreasing
+ * with d_row_ptr[m] == nnz and columns in [0, n).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (on different
+ * streams or threads) race.  Asynchronous on `stream`. */
+int sblas_spmv_csr_noalloc(int m, int n, int64_t nnz, const int *d_row_ptr, const int *d_col,
+                           const void *d_val, int val_type, double alpha, const double *d_x,
+                           double beta, double *d_y, void *stream);
+
 /* Fills opts with defaults (auto kernel, C=1, v1 partition, fused launch). */
 void sblas_opts_default(sblas_opts *opts);
 
