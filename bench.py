@@ -600,7 +600,10 @@ def main():
     ap.add_argument("--workload", default="banded", choices=["banded", "rmat", "powerlaw"])
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--chunks", type=int, default=0)
-    ap.add_argument("--variant", type=int, default=0, help="kernel 1 form: 1 sub-wave, 2 row blocks")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="kernel 1 form: 1 sub-wave, 2 row blocks, 3 column panels, 4 sorted blocks")
+    ap.add_argument("--values", default="f64", choices=["f64", "f32", "bf16"],
+                    help="value type of the timed plan (profiling the narrow forms)")
     ap.add_argument("--items", type=int, default=0)
     ap.add_argument("--subwave", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=-1)
@@ -654,7 +657,14 @@ def main():
         chunks = args.chunks
     rp = torch.from_numpy(A.row_ptr).to(dev)
     col = torch.from_numpy(A.col).to(dev)
-    val = torch.from_numpy(A.val).to(dev)
+    if args.values == "f64":
+        val = torch.from_numpy(A.val).to(dev)
+    elif args.values == "f32":
+        val = torch.from_numpy(A.val.astype(np.float32)).to(dev)
+    else:
+        from hpda_sblas_pnnl_amd import gen as G
+        val = torch.from_numpy(G.to_bf16_bits(A.val).view(np.int16)).to(dev)
+    val_bytes = {"f64": 8, "f32": 4, "bf16": 2}[args.values]
     dx = torch.from_numpy(x).to(dev)
     dy0 = torch.from_numpy(y0).to(dev)
     dy = dy0.clone()
@@ -690,7 +700,7 @@ def main():
 
     ms_step = wall * 1e3 / args.steps
     gflops = 2.0 * A.nnz / (ms_step * 1e-3) / 1e9
-    cb = compulsory_bytes(A.m, A.n, A.nnz, 8, 1, beta != 0)
+    cb = compulsory_bytes(A.m, A.n, A.nnz, val_bytes, 1, beta != 0)
     k_ms = float(np.mean(ktimes)) if len(ktimes) else float("nan")
     achieved = cb / (k_ms * 1e-3) / 1e9
     kstats = ({"min": round(float(np.min(ktimes)), 4), "median": round(float(np.median(ktimes)), 4),
@@ -708,7 +718,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.values == "f64" else f"{args.values} values, f64 accumulate",
         "data": "synthetic (seeded generator)",
         "config": {"workload": desc, "m": A.m, "n": A.n, "nnz": A.nnz, "alpha": alpha,
                    "beta": beta, "chunks": info["chunks"], "kernel": info["kernel"],

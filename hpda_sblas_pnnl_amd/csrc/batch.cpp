@@ -2,6 +2,7 @@
 // with a per-matrix autotuner and hipGraph replay.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -22,6 +23,11 @@ namespace {
 struct Choice {
   int subwave = 16, items = 8, chunks = 1;
 };
+
+int env_int_b(const char *name, int fallback) {
+  const char *e = std::getenv(name);
+  return (e && *e) ? std::atoi(e) : fallback;
+}
 
 int pow2_round_b(double v, int lo, int hi) {
   if (!(v > 0)) return lo;
@@ -60,6 +66,12 @@ struct sblas_batch_s {
   int *d_qbound = nullptr;
   int *d_gmat = nullptr, *d_grow = nullptr, *d_gptr = nullptr, *d_gslot = nullptr;
   int groups = 0;
+  // FP32 copies of x for the narrow-x panel matrices, converted before the
+  // panel launch on every run
+  std::vector<sblas::X32Segment> x32segs;
+  sblas::X32Segment *d_x32segs = nullptr;
+  int64_t x32_longest = 0;
+  int panel_kind = sblas::kBatchPanel;  // kBatchPanelNarrow when every panel matrix is narrow
   int n_panel = 0;                 // items [0, n_panel) are row panels, the rest row blocks
   int grid_panel = 0, grid_rows = 0;
   int items_per_thread = 8;
@@ -110,8 +122,10 @@ struct sblas_batch_s {
       SBLAS_HIP_TRY(hipEventRecord(fork, s));
       SBLAS_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
     }
-    int st = sblas::launch_batch(sblas::kBatchPanel, items_per_thread, grid_panel, n_panel,
-                                 d_items, d_mats, d_partials, d_qbound, d_ctr, s);
+    int st = sblas::launch_batch_x32(static_cast<int>(x32segs.size()), d_x32segs, x32_longest, s);
+    if (st != SBLAS_OK) return st;
+    st = sblas::launch_batch(panel_kind, items_per_thread, grid_panel, n_panel,
+                             d_items, d_mats, d_partials, d_qbound, d_ctr, s);
     if (st != SBLAS_OK) return st;
     st = sblas::launch_batch(sblas::kBatchRowBlock, items_per_thread, grid_rows, n_rows,
                              d_items + n_panel, d_mats, d_partials, d_qbound + Q, d_ctr + Q, rs);
@@ -155,6 +169,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
   };
   std::vector<std::vector<Pending>> pending(count);
   std::vector<std::vector<int>> h_rps(count);
+  bool any_wide_panel = false;  // a row panel with FP64 x in LDS
   for (int i = 0; i < count; i++) {
     const sblas_batch_item &it = items[i];
     if (it.m < 0 || it.n < 0 || it.nnz < 0 || it.nnz > INT32_MAX - 4096 || !it.d_row_ptr ||
@@ -214,11 +229,15 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     b->panels.emplace_back();
     PanelFormat &f = b->panels.back();
     bool panel = false;
+    // narrow values: x staged from an FP32 copy unless asked not to
+    int nxo = opts ? opts->narrow_x : 0;
+    if (nxo == 0) nxo = env_int_b("SBLAS_NARROW_X", 0) > 0 ? env_int_b("SBLAS_NARROW_X", 0) : 2;
+    const bool narrow = it.val_type != SBLAS_VAL_F64 && nxo == 2;
     if (kv == 1 && it.m > 0 && it.nnz > 0 && want != 1 && want != 2 &&
         (want == 3 || rs.max_row <= 8192)) {
       const std::vector<ExecChunk> one{ExecChunk{0, it.nnz, 0, it.m}};
-      int rows = (opts && opts->panel_rows > 0) ? std::min(opts->panel_rows, kBatchPanelRowsMax)
-                                                : kBatchPanelRowsMax;
+      const int rows_max = narrow ? kBatchPanelRowsNarrow : kBatchPanelRowsMax;
+      int rows = (opts && opts->panel_rows > 0) ? std::min(opts->panel_rows, rows_max) : rows_max;
       int st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col,
                            it.d_val, one, h_rp.data(), want == 3, 1, f, &panel, nullptr);
       // slices beyond 16-bit offsets (the batch kernel reads 16-bit ones
@@ -252,6 +271,25 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       M.pcol = f.pcol;
       M.prows = f.rows;
       c.chunks = 1;
+      if (narrow) {
+        const int64_t padded = (static_cast<int64_t>(it.n) + kPanelCols - 1) / kPanelCols *
+                               kPanelCols;
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(float) * std::max<int64_t>(padded, 4)) != hipSuccess) {
+          set_error(SBLAS_ERR_ALLOC, "batch: FP32 x copy allocation failed");
+          return fail(SBLAS_ERR_ALLOC);
+        }
+        f.buffers.push_back(p);
+        if (hipMemset(p, 0, sizeof(float) * std::max<int64_t>(padded, 4)) != hipSuccess) {
+          set_error(SBLAS_ERR_HIP, "batch: FP32 x copy set-up failed");
+          return fail(SBLAS_ERR_HIP);
+        }
+        M.x32 = static_cast<float *>(p);
+        b->x32segs.push_back(X32Segment{it.d_x, static_cast<float *>(p), it.n});
+        b->x32_longest = std::max<int64_t>(b->x32_longest, it.n);
+      } else {
+        any_wide_panel = true;
+      }
     }
     b->mats.push_back(M);
     if (it.m == 0) {
@@ -359,6 +397,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
   int st;
   if ((st = b->upload(&b->d_mats, b->mats)) != SBLAS_OK ||
       (st = b->upload(&b->d_items, b->items)) != SBLAS_OK ||
+      (st = b->upload(&b->d_x32segs, b->x32segs)) != SBLAS_OK ||
       (st = b->upload(&b->d_gmat, gmat)) != SBLAS_OK ||
       (st = b->upload(&b->d_grow, grow)) != SBLAS_OK ||
       (st = b->upload(&b->d_gptr, gptr)) != SBLAS_OK ||
@@ -377,7 +416,8 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
   // three panel workgroups fill a CU's LDS, and the row-block launch would
   // wait for the panel launch to finish (64-matrix batch: 4.87 -> 4.69 ms)
   const int n_rows = static_cast<int>(b->items.size()) - b->n_panel;
-  const int occ_panel = batch_occupancy(kBatchPanel, b->items_per_thread);
+  b->panel_kind = any_wide_panel ? kBatchPanel : kBatchPanelNarrow;
+  const int occ_panel = batch_occupancy(b->panel_kind, b->items_per_thread);
   const int panel_per_cu = (n_rows > 0 && occ_panel > 1) ? occ_panel - 1 : occ_panel;
   b->grid_panel = std::min<int>(b->n_panel, std::max(1, cus) * panel_per_cu);
   b->grid_rows = std::min<int>(n_rows, std::max(1, cus) *

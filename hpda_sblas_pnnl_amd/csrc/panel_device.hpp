@@ -23,6 +23,14 @@ __device__ __forceinline__ double add_rn(double a, double b) {
 
 constexpr int kItems = 8;               // elements per thread per chunk (4 pairs)
 constexpr int kChunk = 256 * kItems;    // elements per chunk of a slice
+// Elements per thread per chunk of the narrow-x body: FP32 products let a
+// chunk of twice the elements fit the LDS of an FP64 chunk, so as many bytes
+// are in flight per workgroup as with FP64 values.
+constexpr int kItemsNarrow = 16;
+template <bool NX>
+__host__ __device__ constexpr int chunk_elems() {
+  return 256 * (NX ? kItemsNarrow : kItems);
+}
 static_assert(kPanelPad % 2 == 0, "pairs never straddle a slice end");
 
 // A slice's row offsets are staged in LDS as 32-bit words, in rounds of 256
@@ -38,7 +46,8 @@ __host__ __device__ constexpr int range_rounds() {
 // With NX (narrow x: FP32 x window and FP32 products) the first two halve.
 template <typename OT, int RPT, bool NX = false>
 __host__ __device__ constexpr int lds_doubles() {
-  return (NX ? (kPanelCols + kChunk) / 2 : kPanelCols + kChunk) + range_rounds<OT, RPT>() * 128;
+  return (NX ? (kPanelCols + chunk_elems<true>()) / 2 : kPanelCols + kChunk) +
+         range_rounds<OT, RPT>() * 128;
 }
 
 // s_waitcnt vmcnt(0) alone.  gfx9-family encoding of the immediate: vmcnt in
@@ -160,12 +169,14 @@ __device__ __forceinline__ void row_panel(
     double *__restrict__ slots, double *lds, const float *__restrict__ x32 = nullptr) {
   constexpr int W = kPanelCols;
   constexpr int NR = range_rounds<OT, RPT>();
+  constexpr int IT = NX ? kItemsNarrow : kItems;  // elements per thread per chunk
+  constexpr int CH = chunk_elems<NX>();           // elements per chunk
   typedef typename RawPair<VT>::type RV;
   double *xs = lds;
   double *prod = lds + W;
   float *xsf = reinterpret_cast<float *>(lds);
   float *prodf = reinterpret_cast<float *>(lds + W / 2);
-  uint32_t *rs = reinterpret_cast<uint32_t *>(lds + (NX ? (W + kChunk) / 2 : W + kChunk));
+  uint32_t *rs = reinterpret_cast<uint32_t *>(lds + (NX ? (W + CH) / 2 : W + CH));
   const int t = threadIdx.x;
   const int wv = t & ~63;  // first thread of this thread's wave
   const int s0 = slice_base[p], s1 = slice_base[p + 1];
@@ -176,8 +187,8 @@ __device__ __forceinline__ void row_panel(
   int b[RPT], e[RPT];     // element range [b, e) of row t + 256 j in the current slice
   int nb[RPT], ne[RPT];   // the same in slice r_next, loaded a whole slice ahead
   int r_next = -1;
-  RV v[kItems / 2];       // the next chunk's value pairs
-  uint32_t cw[kItems / 2];  // and column offset pairs
+  RV v[IT / 2];           // the next chunk's value pairs
+  uint32_t cw[IT / 2];    // and column offset pairs
   uint32_t end2 = 0;      // low word of slice_off at the end of the slice after the current one
   int r_shift = 0;        // first 16-bit offset of the staged slice inside its first word
   double xfix = 0.0;      // x[n - 1] when n is odd and this thread's pair holds it
@@ -279,7 +290,7 @@ __device__ __forceinline__ void row_panel(
   // re-read its last pair (never summed)
   auto load_chunk = [&](const Cursor &c) {
 #pragma unroll
-    for (int i = 0; i < kItems / 2; i++) {
+    for (int i = 0; i < IT / 2; i++) {
       const int64_t k = c.base + min(c.k0 + 512 * i + 2 * t, c.len - 2);
       v[i] = load_pair(pval + k);
       cw[i] = load_cols(pcol + k);
@@ -302,7 +313,7 @@ __device__ __forceinline__ void row_panel(
   while (cur.s < s1) {
     // products of the current chunk (x of its slice is in LDS)
 #pragma unroll
-    for (int i = 0; i < kItems / 2; i++) {
+    for (int i = 0; i < IT / 2; i++) {
       if constexpr (NX) {
         float a0, a1;
         widen_f(v[i], a0, a1);
@@ -320,7 +331,7 @@ __device__ __forceinline__ void row_panel(
       }
     }
     Cursor nxt = cur;
-    nxt.k0 += kChunk;
+    nxt.k0 += CH;
     if (nxt.k0 >= cur.len) {  // the next slice starts where this one ends
       nxt.s = cur.s + 1;
       nxt.k0 = 0;
@@ -345,7 +356,7 @@ __device__ __forceinline__ void row_panel(
     // included) unchanged
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
-      const int lo = max(b[j], cur.k0) - cur.k0, hi = min(e[j], cur.k0 + kChunk) - cur.k0;
+      const int lo = max(b[j], cur.k0) - cur.k0, hi = min(e[j], cur.k0 + CH) - cur.k0;
       if constexpr (NX) {
         // aligned runs of four FP32 products (one 16-byte LDS read), even
         // and odd places into two FP64 sums

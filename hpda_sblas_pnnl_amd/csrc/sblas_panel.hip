@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "internal.hpp"
 #include "panel_device.hpp"
@@ -60,6 +61,27 @@ __global__ __launch_bounds__(256) void k_panel_spmv(
   const PanelInfo pi = info[p];
   panel::row_panel<VT, RPT, OT, NX>(p, n, R, pi, slice_base, slice_off, srp, pval, pcol, x, y,
                                     alpha, beta, partials + 2 * p, lds, x32);
+}
+
+// The narrow-x instances, held to four workgroups per CU (their 36 KB of LDS
+// allow four; registers decide).
+template <typename VT, int RPT, typename OT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_panel_spmv_nx(
+    int n, int R, int npanels, int xcd_remap, const PanelInfo *__restrict__ info,
+    const int *__restrict__ slice_base, const int64_t *__restrict__ slice_off,
+    const OT *__restrict__ srp, const VT *__restrict__ pval,
+    const uint16_t *__restrict__ pcol, const double *__restrict__ x, double *__restrict__ y,
+    double alpha, double beta, double *__restrict__ partials, const float *__restrict__ x32) {
+  __shared__ double lds[panel::lds_doubles<OT, RPT, true>()];
+  int p = blockIdx.x;
+  if (xcd_remap) {
+    const int per = gridDim.x >> 3;
+    p = (p & 7) * per + (p >> 3);
+  }
+  if (p >= npanels) return;  // uniform per workgroup
+  const PanelInfo pi = info[p];
+  panel::row_panel<VT, RPT, OT, true>(p, n, R, pi, slice_base, slice_off, srp, pval, pcol, x, y,
+                                      alpha, beta, partials + 2 * p, lds, x32);
 }
 
 // x32[i] = x[i] rounded to FP32 (the narrow-x column-panel form's x copy)
@@ -114,30 +136,81 @@ __global__ void k_panel_range(int npanels, PanelInfo *__restrict__ info,
   }
 }
 
-// Per row of each panel, element counts in each column panel:
-// cnt[(slice_base[p] + q - q_lo) * R + local_row].
-__global__ void k_panel_count(int npanels, int R, const PanelInfo *__restrict__ info,
-                              const int2 *__restrict__ win, const int *__restrict__ rp,
-                              const int *__restrict__ col, const int *__restrict__ slice_base,
-                              uint32_t *__restrict__ cnt) {
+// Element-parallel passes over one row panel's element window: one
+// workgroup per row panel, 1024 elements per step (four consecutive elements
+// per thread).  An element's row comes from a binary search of the panel's
+// row pointers (clamped to the window) in LDS.  A run is a row's elements in
+// one column panel -- contiguous, since columns ascend within a row -- and
+// starts where the row starts or the column panel changes; an element's place
+// in its run is its distance from the run's start, found by a workgroup
+// max-scan of run starts carried from step to step.
+//   count (SCATTER false): at a run's last element,
+//       cnt[slice * R + row] = run length
+//   scatter (SCATTER true): the element goes to
+//       slice_off[slice] + srp[slice * (R + 1) + row] + place
+//   with its value and its column's offset inside the column panel.
+template <typename VT, bool SCATTER>
+__global__ __launch_bounds__(256) void k_panel_runs(
+    int npanels, int R, const PanelInfo *__restrict__ info, const int2 *__restrict__ win,
+    const int *__restrict__ rp, const int *__restrict__ col, const VT *__restrict__ val,
+    const int *__restrict__ slice_base, const int64_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ srp, uint32_t *__restrict__ cnt, VT *__restrict__ pval,
+    uint16_t *__restrict__ pcol) {
+  constexpr int IT = 4;
+  typedef hipcub::BlockScan<int, 256> Scan;
+  __shared__ typename Scan::TempStorage tmp;
+  __shared__ int lrp[kPanelRowsMax + 1];
   const int p = blockIdx.x;
-  if (p >= npanels) return;
+  if (p >= npanels) return;  // uniform per workgroup
   const PanelInfo pi = info[p];
   const int2 w = win[p];
   const int sb = slice_base[p];
-  for (int rl = threadIdx.x; rl < pi.nrow; rl += blockDim.x) {
-    const int r = pi.row0 + rl;
-    int cur = -1, c = 0;
-    for (int j = max(rp[r], w.x); j < min(rp[r + 1], w.y); j++) {
-      const int q = col[j] / kPanelCols;
-      if (q != cur) {
-        if (cur >= 0) cnt[static_cast<int64_t>(sb + cur - pi.q_lo) * R + rl] = c;
-        cur = q;
-        c = 0;
+  const int t = threadIdx.x;
+  for (int i = t; i <= pi.nrow; i += 256) lrp[i] = min(max(rp[pi.row0 + i], w.x), w.y);
+  __syncthreads();
+  int carry = w.x;  // latest run start before this step (the same in every thread)
+  for (int base = w.x; base < w.y; base += 256 * IT) {
+    int rl[IT], q[IT], c[IT], start[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+      const int j = base + t * IT + k;
+      rl[k] = 0;
+      q[k] = 0;
+      c[k] = 0;
+      start[k] = -1;
+      if (j < w.y) {
+        int lo = 0, hi = pi.nrow - 1;  // the last row whose (clamped) start is <= j
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (lrp[mid] <= j) lo = mid; else hi = mid - 1;
+        }
+        rl[k] = lo;
+        c[k] = col[j];
+        q[k] = c[k] / kPanelCols;
+        const bool first = j == lrp[lo] || col[j - 1] / kPanelCols != q[k];
+        start[k] = first ? j : -1;
       }
-      c++;
     }
-    if (cur >= 0) cnt[static_cast<int64_t>(sb + cur - pi.q_lo) * R + rl] = c;
+    int agg = -1;
+    Scan(tmp).InclusiveScan(start, start, hipcub::Max(), agg);
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+      const int j = base + t * IT + k;
+      if (j < w.y) {
+        const int s = sb + q[k] - pi.q_lo;
+        const int place = j - max(start[k], carry);
+        if constexpr (SCATTER) {
+          const int64_t pos = slice_off[s] + srp[static_cast<int64_t>(s) * (R + 1) + rl[k]] + place;
+          pval[pos] = val[j];
+          pcol[pos] = static_cast<uint16_t>(c[k] - q[k] * kPanelCols);
+        } else {
+          const bool last = j + 1 == lrp[rl[k] + 1] || col[j + 1] / kPanelCols != q[k];
+          if (last) cnt[static_cast<int64_t>(s) * R + rl[k]] = static_cast<uint32_t>(place + 1);
+        }
+      }
+    }
+    carry = max(carry, agg);
+    __syncthreads();  // the scan's storage is reused by the next step
   }
 }
 
@@ -180,38 +253,6 @@ __global__ void k_narrow_u16(int64_t count, const uint32_t *__restrict__ a,
                              uint16_t *__restrict__ b) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < count) b[i] = static_cast<uint16_t>(a[i]);
-}
-
-template <typename VT>
-__global__ void k_panel_scatter(int npanels, int R, const PanelInfo *__restrict__ info,
-                                const int2 *__restrict__ win, const int *__restrict__ rp,
-                                const int *__restrict__ col, const VT *__restrict__ val,
-                                const int *__restrict__ slice_base,
-                                const int64_t *__restrict__ slice_off,
-                                const uint32_t *__restrict__ srp, VT *__restrict__ pval,
-                                uint16_t *__restrict__ pcol) {
-  const int p = blockIdx.x;
-  if (p >= npanels) return;
-  const PanelInfo pi = info[p];
-  const int2 w = win[p];
-  const int sb = slice_base[p];
-  for (int rl = threadIdx.x; rl < pi.nrow; rl += blockDim.x) {
-    const int r = pi.row0 + rl;
-    int cur = -1;
-    int64_t pos = 0;
-    for (int j = max(rp[r], w.x); j < min(rp[r + 1], w.y); j++) {
-      const int c = col[j];
-      const int q = c / kPanelCols;
-      if (q != cur) {
-        cur = q;
-        const int s = sb + q - pi.q_lo;
-        pos = slice_off[s] + srp[static_cast<int64_t>(s) * (R + 1) + rl];
-      }
-      pval[pos] = val[j];
-      pcol[pos] = static_cast<uint16_t>(c - q * kPanelCols);
-      pos++;
-    }
-  }
 }
 
 }  // namespace
@@ -329,8 +370,9 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{1},
                      reinterpret_cast<uint32_t *>(overflow));
   if (f.npanels > 0)
-    hipLaunchKernelGGL(k_panel_count, dim3(f.npanels), dim3(256), 0, s, f.npanels, R, f.info, win,
-                       rp, col, f.slice_base, cnt);
+    hipLaunchKernelGGL((k_panel_runs<double, false>), dim3(f.npanels), dim3(256), 0, s,
+                       f.npanels, R, f.info, win, rp, col, nullptr, f.slice_base, nullptr,
+                       nullptr, cnt, nullptr, nullptr);
   if (f.nslices > 0) {
 #define SBLAS_SCAN_CASE(RPT)                                                                  \
   case RPT:                                                                                   \
@@ -406,19 +448,19 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     const dim3 g(f.npanels), b(256);
     switch (val_type) {
       case SBLAS_VAL_F64:
-        hipLaunchKernelGGL(k_panel_scatter<double>, g, b, 0, s, f.npanels, R, f.info, win, rp, col,
-                           static_cast<const double *>(val), f.slice_base, f.slice_off, f.srp32,
-                           static_cast<double *>(f.pval), f.pcol);
+        hipLaunchKernelGGL((k_panel_runs<double, true>), g, b, 0, s, f.npanels, R, f.info, win, rp,
+                           col, static_cast<const double *>(val), f.slice_base, f.slice_off,
+                           f.srp32, nullptr, static_cast<double *>(f.pval), f.pcol);
         break;
       case SBLAS_VAL_F32:
-        hipLaunchKernelGGL(k_panel_scatter<float>, g, b, 0, s, f.npanels, R, f.info, win, rp, col,
-                           static_cast<const float *>(val), f.slice_base, f.slice_off, f.srp32,
-                           static_cast<float *>(f.pval), f.pcol);
+        hipLaunchKernelGGL((k_panel_runs<float, true>), g, b, 0, s, f.npanels, R, f.info, win, rp,
+                           col, static_cast<const float *>(val), f.slice_base, f.slice_off,
+                           f.srp32, nullptr, static_cast<float *>(f.pval), f.pcol);
         break;
       default:
-        hipLaunchKernelGGL(k_panel_scatter<uint16_t>, g, b, 0, s, f.npanels, R, f.info, win, rp,
-                           col, static_cast<const uint16_t *>(val), f.slice_base, f.slice_off,
-                           f.srp32, static_cast<uint16_t *>(f.pval), f.pcol);
+        hipLaunchKernelGGL((k_panel_runs<uint16_t, true>), g, b, 0, s, f.npanels, R, f.info, win,
+                           rp, col, static_cast<const uint16_t *>(val), f.slice_base, f.slice_off,
+                           f.srp32, nullptr, static_cast<uint16_t *>(f.pval), f.pcol);
         break;
     }
   }
@@ -444,10 +486,16 @@ int panel_launch_typed(const PanelFormat &f, const OT *srp, int m, int n, int xc
   const int grid = xcd_remap ? ((f.npanels + 7) / 8) * 8 : f.npanels;
 #define SBLAS_PANEL_CASE(RPT)                                                                 \
   case RPT:                                                                                   \
-    hipLaunchKernelGGL((k_panel_spmv<VT, RPT, OT, NX>), dim3(grid), dim3(256), 0, s, n,        \
-                       f.rows, f.npanels, xcd_remap, f.info, f.slice_base, f.slice_off, srp,  \
-                       static_cast<const VT *>(f.pval), f.pcol, x, y, alpha, beta,           \
-                       f.partials, f.x32);                                                    \
+    if constexpr (NX)                                                                         \
+      hipLaunchKernelGGL((k_panel_spmv_nx<VT, RPT, OT>), dim3(grid), dim3(256), 0, s, n,       \
+                         f.rows, f.npanels, xcd_remap, f.info, f.slice_base, f.slice_off,     \
+                         srp, static_cast<const VT *>(f.pval), f.pcol, x, y, alpha, beta,     \
+                         f.partials, f.x32);                                                  \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_panel_spmv<VT, RPT, OT, false>), dim3(grid), dim3(256), 0, s, n,  \
+                         f.rows, f.npanels, xcd_remap, f.info, f.slice_base, f.slice_off,     \
+                         srp, static_cast<const VT *>(f.pval), f.pcol, x, y, alpha, beta,     \
+                         f.partials, nullptr);                                                \
     break;
   switch (f.rpt) {
     SBLAS_PANEL_CASE(1)
@@ -476,11 +524,13 @@ int panel_launch_nx(const PanelFormat &f, int m, int n, int xcd_remap, const dou
 template <typename VT>
 int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const double *x, double *y,
                     double alpha, double beta, hipStream_t s) {
-  if (f.x32) {
-    // the FP32 copy of x first, on the same stream
-    hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, f.x32);
-    SBLAS_HIP_TRY(hipGetLastError());
-    return panel_launch_nx<VT, true>(f, m, n, xcd_remap, x, y, alpha, beta, s);
+  if constexpr (!std::is_same<VT, double>::value) {
+    if (f.x32) {
+      // the FP32 copy of x first, on the same stream
+      hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, f.x32);
+      SBLAS_HIP_TRY(hipGetLastError());
+      return panel_launch_nx<VT, true>(f, m, n, xcd_remap, x, y, alpha, beta, s);
+    }
   }
   return panel_launch_nx<VT, false>(f, m, n, xcd_remap, x, y, alpha, beta, s);
 }
@@ -499,10 +549,10 @@ int panel_rows_for(int val_type, int m, int device, bool narrow_x) {
       val_type == SBLAS_VAL_F64
           ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t, false>)
       : val_type == SBLAS_VAL_F32
-          ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t, true>)
+          ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_spmv_nx<float, 8, uint16_t>)
                       : reinterpret_cast<const void *>(&k_panel_spmv<float, 8, uint16_t, false>))
           : (narrow_x
-                 ? reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t, true>)
+                 ? reinterpret_cast<const void *>(&k_panel_spmv_nx<uint16_t, 8, uint16_t>)
                  : reinterpret_cast<const void *>(&k_panel_spmv<uint16_t, 8, uint16_t, false>));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
       per_cu < 1)

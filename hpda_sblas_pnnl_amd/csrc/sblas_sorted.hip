@@ -20,6 +20,7 @@
 #include <climits>
 
 #include "internal.hpp"
+#include "rows_device.hpp"
 
 namespace sblas {
 namespace {
@@ -46,7 +47,7 @@ __device__ __forceinline__ double lanes_sum(double v) {
 }
 
 template <typename VT, int SUBW>
-__global__ __launch_bounds__(B) void k_csr_sorted(
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_csr_sorted(
     const RowBlock *__restrict__ blocks, int nblocks, int xcd_remap,
     const int *__restrict__ rp, const int *__restrict__ scol, const VT *__restrict__ sval,
     const uint16_t *__restrict__ spos, const double *__restrict__ x, double *__restrict__ y,
@@ -132,16 +133,9 @@ __global__ __launch_bounds__(B) void k_csr_sorted(
     if (t + k * B <= nrows) rps[t + k * B] = rpv[k];
   __syncthreads();
 
-  // row sums in element order, one sub-wave of SUBW lanes per row
-  constexpr int SUBS = B / SUBW;
-  const int sub = t / SUBW, lane = t % SUBW;
-  for (int i = sub; i < nrows; i += SUBS) {
-    const int s0 = max(rps[i], lo) - lo, e0 = min(rps[i + 1], hi) - lo;
-    double s = 0.0;
-    for (int j = s0 + lane; j < e0; j += SUBW) s += prod[j];
-    if constexpr (SUBW > 1) s = lanes_sum<SUBW>(s);
-    if (lane == 0) ysh[i] = s;
-  }
+  // row sums in element order: one sub-wave of SUBW lanes per row, whole
+  // waves for long rows
+  rows::block_row_sums<SUBW, B>(prod, rps, nrows, lo, hi, ysh);
   __syncthreads();
 
   // y for whole rows (consecutive threads on consecutive rows), partials for

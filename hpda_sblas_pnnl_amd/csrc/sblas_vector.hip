@@ -16,6 +16,7 @@
 // they do not evict x from L2 / MALL.  Products and sums are FP64 for every
 // stored value type.
 #include "internal.hpp"
+#include "rows_device.hpp"
 
 namespace sblas {
 namespace {
@@ -240,16 +241,8 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
     if (t + k * BLOCK <= nrows) rps[t + k * BLOCK] = rpv[k];
   __syncthreads();
 
-  // phase 2: one sub-wave of SUBW lanes per row
-  constexpr int SUBS = BLOCK / SUBW;
-  const int sub = t / SUBW, lane = t % SUBW;
-  for (int i = sub; i < nrows; i += SUBS) {
-    const int s0 = max(rps[i], lo) - lo, e0 = min(rps[i + 1], hi) - lo;
-    double s = 0.0;
-    for (int j = s0 + lane; j < e0; j += SUBW) s += prod[j];
-    if constexpr (SUBW > 1) s = subwave_sum<SUBW>(s);
-    if (lane == 0) ysh[i] = s;
-  }
+  // phase 2: one sub-wave of SUBW lanes per row, whole waves for long rows
+  rows::block_row_sums<SUBW, BLOCK>(prod, rps, nrows, lo, hi, ysh);
   __syncthreads();
 
   // phase 3: y for whole rows, partials for rows cut by the window

@@ -68,7 +68,8 @@ class SpmvBatch:
     pick row blocks or column panels per matrix, 2 forces row blocks, 3 asks
     for column panels wherever the format allows them."""
 
-    def __init__(self, items, subwave=0, chunks=0, item_size=0, variant=0, panel_rows=0):
+    def __init__(self, items, subwave=0, chunks=0, item_size=0, variant=0, panel_rows=0,
+                 narrow_x=0):
         self._keep = []
         arr = (_lib.BatchItem * max(len(items), 1))()
         for i, it in enumerate(items):
@@ -85,7 +86,7 @@ class SpmvBatch:
             arr[i].alpha = float(it.get("alpha", 1.0))
             arr[i].beta = float(it.get("beta", 0.0))
         o = make_opts(subwave=subwave, chunks=chunks, items=item_size, variant=variant,
-                      panel_rows=panel_rows)
+                      panel_rows=panel_rows, narrow_x=narrow_x)
         h = ctypes.c_void_p()
         check(lib().sblas_batch_create(len(items), arr, ctypes.byref(o), ctypes.byref(h)),
               "sblas_batch_create")

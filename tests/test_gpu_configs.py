@@ -306,7 +306,16 @@ def test_batch64_fp32_graph(torch_dev, oracle):
         batch.graph_launch()
         torch.cuda.synchronize()
         outs.append([it["y"].cpu().numpy() for it in items])
-    for i in range(64):
-        assert oracle.err_normwise(outs[0][i], refs32[i]) <= TOL, i
+    for i, (kind, A) in enumerate(mats):
+        if batch.choice(i)["variant"] == 3:
+            # column panels stage x in FP32: each row within two FP32
+            # roundings per product of the FP32-value oracle
+            vr = A.val.astype(np.float32).astype(np.float64)
+            x = G.values(A.n, i, 2, "uniform")
+            beta = 0.5 if i % 3 == 0 else 0.0
+            assert oracle.err_maxrow(A.m, vr, A.row_ptr, A.col, x, 1.0, beta,
+                                     y0s[i].cpu().numpy(), outs[0][i], refs32[i]) <= 2.0**-22, i
+        else:
+            assert oracle.err_normwise(outs[0][i], refs32[i]) <= TOL, i
         assert oracle.err_normwise(outs[0][i], refs64[i]) <= TOL_FP32_VS_FP64, i
         assert np.array_equal(outs[0][i], outs[1][i]) and np.array_equal(outs[0][i], outs[2][i])
