@@ -64,7 +64,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("kernel", _I), ("chunks", _I), ("part_mode", _I), ("nb", _I),
                 ("streams", _I), ("subwave", _I), ("sigma", _I), ("xcd_remap", _I),
                 ("variant", _I), ("items", _I), ("panel_rows", _I), ("narrow_x", _I),
-                ("reserved", _I * 4)]
+                ("panel_layout", _I), ("reserved", _I * 3)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -74,7 +74,8 @@ class PlanInfo(ctypes.Structure):
                 ("mean_row", _D), ("cv_row", _D), ("max_row", _I64), ("launches", _I),
                 ("variant", _I), ("row_blocks", _I), ("items", _I),
                 ("panel_rows", _I), ("panel_slices", _I), ("tuned", _I),
-                ("trial_ms", ctypes.c_float * 3)]
+                ("trial_ms", ctypes.c_float * 3), ("panel_layout", _I), ("narrow_x", _I),
+                ("panel_elements", _I64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "trial_ms"}

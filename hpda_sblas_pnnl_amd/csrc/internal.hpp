@@ -175,6 +175,9 @@ int launch_tile_fixup(int tiles, const TileCarry *carry, double *y, double alpha
 constexpr int kPanelCols = 4096;  // x panel staged in LDS (32 KB of FP64)
 constexpr int kPanelRowsMax = 2048;  // rows per row panel (8 per thread)
 constexpr int kPanelPad = 8;         // slices start on 8-element boundaries
+constexpr int kPanelRowOrder = 0;    // column-panel slice layouts
+constexpr int kPanelSorted = 1;
+constexpr uint16_t kSortedPad = 0xFFFF;  // column offset of a padding element
 // One row panel: rows [row0, row0 + nrow) of one execution chunk, its first
 // column panel, and flags: bit 0 = the first row is cut by the chunk's
 // element window (partial slot 2p), bit 1 = the last row is (slot 2p+1).
@@ -197,6 +200,16 @@ struct PanelFormat {
   uint32_t *srp32 = nullptr;
   void *pval = nullptr;         // values, slice-major, slices padded to kPanelPad
   uint16_t *pcol = nullptr;     // column offsets inside the column panel (padding: 0)
+  // layout inside a slice: kPanelRowOrder (rows in order, row offsets srp) or
+  // kPanelSorted (rows ordered by their element count in the slice, longest
+  // first, in groups of 64: perm holds the R local rows of each slice in
+  // that order, goff the groups' element offsets inside the slice, groups+1
+  // per slice; group g stores element k of its i-th row at goff[g] + 64k + i,
+  // rows shorter than the group's longest padded with column kSortedPad)
+  int layout = 0;
+  int groups = 0;
+  uint16_t *perm = nullptr;
+  uint32_t *goff = nullptr;
   double *partials = nullptr;   // 2 slots per row panel
   // narrow-x form (FP32 / BF16 values): FP32 copy of x written before each
   // launch, padded with zeros to whole column panels; nullptr: FP64 x
@@ -207,15 +220,18 @@ struct PanelFormat {
 };
 int panel_rpt(int rows);
 // Rows per row panel so the panels fill whole rounds of resident workgroups.
-int panel_rows_for(int val_type, int m, int device, bool narrow_x = false);
+int panel_rows_for(int val_type, int m, int device, bool narrow_x = false,
+                   int layout = kPanelRowOrder, int chunks = 1);
 // Builds the format on the device: each execution chunk of `ex` cut into row
 // panels of at most `rows` rows (h_rp: host copy of the row pointers).  With
 // force == false it stops after the per-panel column ranges and sets
 // *chosen = false when the form would not help (few row panels, sparse
 // slices, or rows whose x window is narrow).
+// layout kPanelSorted rounds the rows per panel up to a multiple of 64.
 int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp, const int *col,
                 const void *val, const std::vector<ExecChunk> &ex, const int *h_rp,
-                bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s);
+                bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s,
+                int layout = kPanelRowOrder);
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
                       const double *x, double *y, double alpha, double beta, hipStream_t s);
 
@@ -243,6 +259,10 @@ struct BatchMatrix {
   // zeros to whole column panels, written before the panel launch; nullptr:
   // x staged in FP64
   const float *x32;
+  // sorted-group slice layout (the narrow-x panels): row order and group
+  // offsets of each slice; nullptr: rows in order (srp)
+  const uint16_t *perm;
+  const uint32_t *goff;
 };
 // One matrix's x to convert for narrow-x panels: x32[0, n) = x[0, n) in FP32
 // (one grid row of the conversion launch per segment; longest = the largest n).
@@ -254,8 +274,9 @@ struct X32Segment {
 int launch_batch_x32(int nseg, const X32Segment *segs, int64_t longest, hipStream_t s);
 constexpr int kBatchRowBlock = 0;
 constexpr int kBatchPanel = 1;
-// launch kind for a batch whose row panels all stage x in FP32: a kernel with
-// the narrow-x bodies only, at most kBatchPanelRowsNarrow rows per panel
+// launch kind for a batch whose row panels all stage x in FP32 (stored in
+// the sorted-group layout): a kernel with that body only, at most
+// kBatchPanelRowsNarrow rows per panel
 constexpr int kBatchPanelNarrow = 2;
 constexpr int kBatchPanelRowsNarrow = 1024;
 struct BatchItem {  // one row block, or one row panel, of one matrix

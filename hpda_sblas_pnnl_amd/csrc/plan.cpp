@@ -784,11 +784,21 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
     int nxo = opts.narrow_x;
     if (nxo == 0) nxo = env_int("SBLAS_NARROW_X", 0) > 0 ? env_int("SBLAS_NARROW_X", 0) : 2;
     const bool narrow_x = val_type != SBLAS_VAL_F64 && nxo == 2;
+    // slice layout: sorted groups where x is staged in FP32, rows in order
+    // otherwise, unless asked (SBLAS_PANEL_LAYOUT overrides the auto setting)
+    int lo_opt = opts.panel_layout;
+    if (lo_opt == 0) lo_opt = env_int("SBLAS_PANEL_LAYOUT", 0);
+    const int layout = lo_opt == 2   ? kPanelSorted
+                       : lo_opt == 1 ? kPanelRowOrder
+                       : narrow_x    ? kPanelSorted
+                                     : kPanelRowOrder;
     const int rows =
-        opts.panel_rows > 0 ? opts.panel_rows : panel_rows_for(val_type, m, device, narrow_x);
+        opts.panel_rows > 0
+            ? opts.panel_rows
+            : panel_rows_for(val_type, m, device, narrow_x, layout, static_cast<int>(exec.size()));
     bool chosen = false;
     st = panel_build(val_type, rows, m, n, nnz, a0, a1, val, exec, h_rp.data(),
-                     opts.variant == 3, min_panels, panel, &chosen, nullptr);
+                     opts.variant == 3, min_panels, panel, &chosen, nullptr, layout);
     if (st == SBLAS_ERR_UNSUPPORTED && opts.variant == 0) {
       // a slice too large for 16-bit offsets: keep the row-block form
       for (void *p : panel.buffers) (void)hipFree(p);
@@ -1226,6 +1236,9 @@ int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info) {
   info->panel_slices = p.variant == 3 ? p.panel.nslices : 0;
   info->tuned = p.tuned;
   for (int i = 0; i < 3; i++) info->trial_ms[i] = p.trial_ms[i];
+  info->panel_layout = p.variant == 3 ? (p.panel.layout == sblas::kPanelSorted ? 2 : 1) : 0;
+  info->narrow_x = p.variant == 3 && p.panel.x32 ? 1 : 0;
+  info->panel_elements = p.variant == 3 ? p.panel.padded : 0;
   if (p.variant == 2) info->subwave = p.sl.subwave;
   return SBLAS_OK;
 }

@@ -14,6 +14,7 @@
 // partials that one merge launch combines in item order.
 #include "internal.hpp"
 #include "panel_device.hpp"
+#include "sorted_panel_device.hpp"
 
 namespace sblas {
 namespace {
@@ -230,6 +231,24 @@ __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchM
   }
 }
 
+// One row panel of a matrix whose x is staged from its FP32 copy (FP32 or
+// BF16 values) in the sorted-group layout, at most kBatchPanelRowsNarrow rows.
+__device__ __forceinline__ void row_panel_narrow(const BatchItem &it, const BatchMatrix &M,
+                                                 double *__restrict__ partials, double *lds) {
+  constexpr int RPT = kBatchPanelRowsNarrow / 256;
+  const int p = it.row_begin;
+  const PanelInfo pi = M.pinfo[p];
+  double *slots = partials + it.pslot;
+  if (M.val_type == SBLAS_VAL_F32)
+    sorted_panel::panel<float, RPT, true>(p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.perm,
+                                          M.goff, static_cast<const float *>(M.pval), M.pcol,
+                                          M.x, M.x32, M.y, M.alpha, M.beta, slots, lds);
+  else
+    sorted_panel::panel<uint16_t, RPT, true>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
+                                             M.perm, M.goff, static_cast<const uint16_t *>(M.pval),
+                                             M.pcol, M.x, M.x32, M.y, M.alpha, M.beta, slots, lds);
+}
+
 // One row panel of a matrix in column-panel form (16-bit slice offsets).
 __device__ __forceinline__ void row_panel_any(const BatchItem &it, const BatchMatrix &M,
                                               double *__restrict__ partials, double *lds) {
@@ -244,27 +263,14 @@ __device__ __forceinline__ void row_panel_any(const BatchItem &it, const BatchMa
                                               M.x, M.y, M.alpha, M.beta, slots, lds);
       break;
     case SBLAS_VAL_F32:
-      if (M.x32)
-        panel::row_panel<float, RPT, uint16_t, true>(
-            p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
-            static_cast<const float *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta, slots, lds,
-            M.x32);
-      else
-        panel::row_panel<float, RPT, uint16_t>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
-                                               M.srp, static_cast<const float *>(M.pval), M.pcol,
-                                               M.x, M.y, M.alpha, M.beta, slots, lds);
+      panel::row_panel<float, RPT, uint16_t>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
+                                             M.srp, static_cast<const float *>(M.pval), M.pcol,
+                                             M.x, M.y, M.alpha, M.beta, slots, lds);
       break;
     default:
-      if (M.x32)
-        panel::row_panel<uint16_t, RPT, uint16_t, true>(
-            p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
-            static_cast<const uint16_t *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta, slots, lds,
-            M.x32);
-      else
-        panel::row_panel<uint16_t, RPT, uint16_t>(p, M.n, M.prows, pi, M.slice_base,
-                                                  M.slice_off, M.srp,
-                                                  static_cast<const uint16_t *>(M.pval), M.pcol,
-                                                  M.x, M.y, M.alpha, M.beta, slots, lds);
+      panel::row_panel<uint16_t, RPT, uint16_t>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
+                                                M.srp, static_cast<const uint16_t *>(M.pval),
+                                                M.pcol, M.x, M.y, M.alpha, M.beta, slots, lds);
       break;
   }
 }
@@ -283,26 +289,6 @@ __device__ __forceinline__ void rows_typed(int w, const BatchItem &it, const Bat
   }
 }
 
-// One row panel of a matrix whose x is staged from its FP32 copy (FP32 or
-// BF16 values), at most kBatchPanelRowsNarrow rows.
-__device__ __forceinline__ void row_panel_narrow(const BatchItem &it, const BatchMatrix &M,
-                                                 double *__restrict__ partials, double *lds) {
-  constexpr int RPT = kBatchPanelRowsNarrow / 256;
-  const int p = it.row_begin;
-  const PanelInfo pi = M.pinfo[p];
-  double *slots = partials + it.pslot;
-  if (M.val_type == SBLAS_VAL_F32)
-    panel::row_panel<float, RPT, uint16_t, true>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
-                                                 M.srp, static_cast<const float *>(M.pval),
-                                                 M.pcol, M.x, M.y, M.alpha, M.beta, slots, lds,
-                                                 M.x32);
-  else
-    panel::row_panel<uint16_t, RPT, uint16_t, true>(
-        p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.srp,
-        static_cast<const uint16_t *>(M.pval), M.pcol, M.x, M.y, M.alpha, M.beta, slots, lds,
-        M.x32);
-}
-
 // The persistent loop of one item kind (MODE kBatchPanel: row panels,
 // kBatchPanelNarrow: row panels that all stage x in FP32, else row blocks).
 // Each kind is its own kernel compiled with only its own body, so its
@@ -316,9 +302,10 @@ __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
                                            int *__restrict__ ctr) {
   static_assert(BLOCK == 256, "the panel body assumes 256 threads");
   constexpr int LDS =
-      MODE == kBatchPanel         ? panel::lds_doubles<uint16_t, kBatchPanelRowsMax / 256>()
-      : MODE == kBatchPanelNarrow ? panel::lds_doubles<uint16_t, kBatchPanelRowsNarrow / 256, true>()
-                                  : BLOCK * ITEMS + kStreamRowsMax;
+      MODE == kBatchPanel ? panel::lds_doubles<uint16_t, kBatchPanelRowsMax / 256>()
+      : MODE == kBatchPanelNarrow
+          ? sorted_panel::lds_doubles<kBatchPanelRowsNarrow / 256, true>()
+          : BLOCK * ITEMS + kStreamRowsMax;
   __shared__ double lds[LDS];
   __shared__ double wsum[BLOCK / 64];
   __shared__ int next;

@@ -31,13 +31,14 @@
 
 #include "internal.hpp"
 #include "panel_device.hpp"
+#include "sorted_panel_device.hpp"
 
 namespace sblas {
 namespace {
 
-__global__ void k_fill_u32(int64_t count, uint32_t *__restrict__ a) {
+__global__ void k_fill_u32(int64_t count, uint32_t *__restrict__ a, uint32_t v) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < count) a[i] = 0u;
+  if (i < count) a[i] = v;
 }
 
 }  // namespace
@@ -82,6 +83,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const PanelInfo pi = info[p];
   panel::row_panel<VT, RPT, OT, true>(p, n, R, pi, slice_base, slice_off, srp, pval, pcol, x, y,
                                       alpha, beta, partials + 2 * p, lds, x32);
+}
+
+// The sorted-group layout: one lane per row of a 64-row group.
+// At most 128 registers a lane: four workgroups per CU.
+template <typename VT, int RPT, bool NX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_panel_sorted(
+    int n, int R, int npanels, int xcd_remap, const PanelInfo *__restrict__ info,
+    const int *__restrict__ slice_base, const int64_t *__restrict__ slice_off,
+    const uint16_t *__restrict__ perm, const uint32_t *__restrict__ goff,
+    const VT *__restrict__ pval, const uint16_t *__restrict__ pcol, const double *__restrict__ x,
+    const float *__restrict__ x32, double *__restrict__ y, double alpha, double beta,
+    double *__restrict__ partials) {
+  __shared__ double lds[sorted_panel::lds_doubles<RPT, NX>()];
+  int p = blockIdx.x;
+  if (xcd_remap) {
+    const int per = gridDim.x >> 3;
+    p = (p & 7) * per + (p >> 3);
+  }
+  if (p >= npanels) return;  // uniform per workgroup
+  const PanelInfo pi = info[p];
+  sorted_panel::panel<VT, RPT, NX>(p, n, R, pi, slice_base, slice_off, perm, goff, pval, pcol, x,
+                                   x32, y, alpha, beta, partials + 2 * p, lds);
 }
 
 // x32[i] = x[i] rounded to FP32 (the narrow-x column-panel form's x copy)
@@ -148,14 +171,18 @@ __global__ void k_panel_range(int npanels, PanelInfo *__restrict__ info,
 //       cnt[slice * R + row] = run length
 //   scatter (SCATTER true): the element goes to
 //       slice_off[slice] + srp[slice * (R + 1) + row] + place
+//   or, in the sorted-group layout (SORTED), with i = rank[slice * R + row]
+//   (the row's place in the slice's order) and G = R / 64 groups,
+//       slice_off[slice] + goff[slice * (G + 1) + i / 64] + 64 * place + i % 64
 //   with its value and its column's offset inside the column panel.
-template <typename VT, bool SCATTER>
+template <typename VT, bool SCATTER, bool SORTED = false>
 __global__ __launch_bounds__(256) void k_panel_runs(
     int npanels, int R, const PanelInfo *__restrict__ info, const int2 *__restrict__ win,
     const int *__restrict__ rp, const int *__restrict__ col, const VT *__restrict__ val,
     const int *__restrict__ slice_base, const int64_t *__restrict__ slice_off,
     const uint32_t *__restrict__ srp, uint32_t *__restrict__ cnt, VT *__restrict__ pval,
-    uint16_t *__restrict__ pcol) {
+    uint16_t *__restrict__ pcol, const uint16_t *__restrict__ rank = nullptr,
+    const uint32_t *__restrict__ goff = nullptr) {
   constexpr int IT = 4;
   typedef hipcub::BlockScan<int, 256> Scan;
   __shared__ typename Scan::TempStorage tmp;
@@ -200,7 +227,14 @@ __global__ __launch_bounds__(256) void k_panel_runs(
         const int s = sb + q[k] - pi.q_lo;
         const int place = j - max(start[k], carry);
         if constexpr (SCATTER) {
-          const int64_t pos = slice_off[s] + srp[static_cast<int64_t>(s) * (R + 1) + rl[k]] + place;
+          int64_t pos;
+          if constexpr (SORTED) {
+            const int i = rank[static_cast<int64_t>(s) * R + rl[k]];
+            pos = slice_off[s] + goff[static_cast<int64_t>(s) * ((R >> 6) + 1) + (i >> 6)] +
+                  64 * place + (i & 63);
+          } else {
+            pos = slice_off[s] + srp[static_cast<int64_t>(s) * (R + 1) + rl[k]] + place;
+          }
           pval[pos] = val[j];
           pcol[pos] = static_cast<uint16_t>(c[k] - q[k] * kPanelCols);
         } else {
@@ -249,6 +283,57 @@ __global__ void k_panel_slice_scan(int nslices, int R, const uint32_t *__restric
   }
 }
 
+// Sorted-group layout: each slice's R rows ordered by their count, longest
+// first (ties in row order), one workgroup per slice; thread t holds places
+// t*RPT .. t*RPT+RPT-1 of the order.  perm[slice * R + i] = the row at place
+// i, rank[slice * R + row] = its place, goff[slice * (G + 1) + g] = 64 times
+// the summed lengths of groups before g (a group's length is its first
+// row's count), snnz[slice] = the slice's stored elements.
+template <int RPT>
+__global__ __launch_bounds__(256) void k_sorted_slice_order(int nslices, int R,
+                                                            const uint32_t *__restrict__ cnt,
+                                                            uint16_t *__restrict__ perm,
+                                                            uint16_t *__restrict__ rank,
+                                                            uint32_t *__restrict__ goff,
+                                                            int64_t *__restrict__ snnz) {
+  typedef hipcub::BlockRadixSort<uint32_t, 256, RPT, uint16_t> Sort;
+  __shared__ typename Sort::TempStorage tmp;
+  __shared__ uint32_t glen[kPanelRowsMax / 64];
+  const int s = blockIdx.x;
+  if (s >= nslices) return;  // uniform per workgroup
+  const int t = threadIdx.x, G = R >> 6;
+  uint32_t key[RPT];
+  uint16_t row[RPT];
+#pragma unroll
+  for (int j = 0; j < RPT; j++) {
+    const int i = t * RPT + j;
+    // places past R sort after every row (count 0, stable order)
+    key[j] = i < R ? cnt[static_cast<int64_t>(s) * R + i] : 0u;
+    row[j] = static_cast<uint16_t>(i);
+  }
+  // counts are at most kPanelCols (13 bits)
+  Sort(tmp).SortDescending(key, row, 0, 13);
+#pragma unroll
+  for (int j = 0; j < RPT; j++) {
+    const int i = t * RPT + j;
+    if (i < R) {
+      perm[static_cast<int64_t>(s) * R + i] = row[j];
+      rank[static_cast<int64_t>(s) * R + row[j]] = static_cast<uint16_t>(i);
+      if ((i & 63) == 0) glen[i >> 6] = key[j];
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    uint32_t off = 0;
+    for (int g = 0; g < G; g++) {
+      goff[static_cast<int64_t>(s) * (G + 1) + g] = off;
+      off += 64u * glen[g];
+    }
+    goff[static_cast<int64_t>(s) * (G + 1) + G] = off;
+    snnz[s] = off;  // a multiple of 64: slices start aligned
+  }
+}
+
 __global__ void k_narrow_u16(int64_t count, const uint32_t *__restrict__ a,
                              uint16_t *__restrict__ b) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -259,7 +344,8 @@ __global__ void k_narrow_u16(int64_t count, const uint32_t *__restrict__ a,
 
 int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp, const int *col,
                 const void *val, const std::vector<ExecChunk> &ex, const int *h_rp,
-                bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s) {
+                bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s,
+                int layout) {
   (void)n;
   (void)m;
   *chosen = false;
@@ -267,7 +353,10 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     set_error(SBLAS_ERR_ARGS, "panel format: rows per panel must be in [1, 2048]");
     return SBLAS_ERR_ARGS;
   }
-  const int R = rows;
+  const bool sorted = layout == kPanelSorted;
+  const int R = sorted ? (rows + 63) / 64 * 64 : rows;  // whole groups of 64 rows
+  f.layout = sorted ? kPanelSorted : kPanelRowOrder;
+  f.groups = R / 64;
   f.rows = R;
   f.rpt = panel_rpt(R);
   f.buffers.clear();
@@ -354,26 +443,49 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   const int64_t ns = std::max<int64_t>(f.nslices, 1);
   uint32_t *cnt = static_cast<uint32_t *>(alloc(sizeof(uint32_t) * ns * R));
   f.srp = nullptr;
-  f.srp32 = static_cast<uint32_t *>(alloc(sizeof(uint32_t) * ns * (R + 1)));
+  f.srp32 = nullptr;
+  uint16_t *rank = nullptr;
+  if (sorted) {
+    f.perm = static_cast<uint16_t *>(alloc(sizeof(uint16_t) * ns * R));
+    rank = static_cast<uint16_t *>(alloc(sizeof(uint16_t) * ns * R));
+    f.goff = static_cast<uint32_t *>(alloc(sizeof(uint32_t) * ns * (f.groups + 1)));
+  } else {
+    f.srp32 = static_cast<uint32_t *>(alloc(sizeof(uint32_t) * ns * (R + 1)));
+  }
   int64_t *snnz = static_cast<int64_t *>(alloc(sizeof(int64_t) * (ns + 1)));
   f.slice_off = static_cast<int64_t *>(alloc(sizeof(int64_t) * (ns + 1)));
   int *overflow = static_cast<int *>(alloc(sizeof(int)));
   f.partials = static_cast<double *>(alloc(sizeof(double) * 2 * np));
-  if (!cnt || !f.srp32 || !snnz || !f.slice_off || !overflow || !f.partials) {
+  if (!cnt || (sorted ? (!f.perm || !rank || !f.goff) : !f.srp32) || !snnz || !f.slice_off ||
+      !overflow || !f.partials) {
     set_error(SBLAS_ERR_ALLOC, "panel format: allocation failed");
     return SBLAS_ERR_ALLOC;
   }
   // counts start at zero: written by a kernel (no memset nodes)
   const int64_t cnt_n = ns * R;
   hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((cnt_n + 255) / 256)), dim3(256), 0, s,
-                     cnt_n, cnt);
+                     cnt_n, cnt, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{1},
-                     reinterpret_cast<uint32_t *>(overflow));
+                     reinterpret_cast<uint32_t *>(overflow), 0u);
   if (f.npanels > 0)
     hipLaunchKernelGGL((k_panel_runs<double, false>), dim3(f.npanels), dim3(256), 0, s,
                        f.npanels, R, f.info, win, rp, col, nullptr, f.slice_base, nullptr,
                        nullptr, cnt, nullptr, nullptr);
-  if (f.nslices > 0) {
+  if (f.nslices > 0 && sorted) {
+#define SBLAS_ORDER_CASE(RPT)                                                                 \
+  case RPT:                                                                                   \
+    hipLaunchKernelGGL(k_sorted_slice_order<RPT>, dim3(f.nslices), dim3(256), 0, s, f.nslices, \
+                       R, cnt, f.perm, rank, f.goff, snnz);                                   \
+    break;
+    switch (f.rpt) {
+      SBLAS_ORDER_CASE(1)
+      SBLAS_ORDER_CASE(2)
+      SBLAS_ORDER_CASE(4)
+      SBLAS_ORDER_CASE(6)
+      SBLAS_ORDER_CASE(8)
+    }
+#undef SBLAS_ORDER_CASE
+  } else if (f.nslices > 0) {
 #define SBLAS_SCAN_CASE(RPT)                                                                  \
   case RPT:                                                                                   \
     hipLaunchKernelGGL(k_panel_slice_scan<RPT>, dim3(f.nslices), dim3(256), 0, s, f.nslices, R, \
@@ -391,7 +503,7 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   int h_over = 0;
   SBLAS_HIP_TRY(hipMemcpyAsync(&h_over, overflow, sizeof(int), hipMemcpyDeviceToHost, s));
   SBLAS_HIP_TRY(hipStreamSynchronize(s));
-  if (!h_over) {
+  if (!h_over && !sorted) {
     // every slice fits 16-bit offsets: keep the narrow copy only
     // two entries of slack: the kernel stages a slice's offsets as whole
     // 32-bit words, and the last slice's last word may end past them
@@ -415,7 +527,7 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     }
     // snnz[nslices] must be 0 for the trailing total
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{2},
-                       reinterpret_cast<uint32_t *>(snnz + f.nslices));
+                       reinterpret_cast<uint32_t *>(snnz + f.nslices), 0u);
     SBLAS_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, snnz, f.slice_off, f.nslices + 1, s));
   }
   // element arrays: slices padded to kPanelPad elements, padding zeroed
@@ -426,7 +538,7 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     SBLAS_HIP_TRY(hipStreamSynchronize(s));
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, int64_t{2},
-                       reinterpret_cast<uint32_t *>(f.slice_off));
+                       reinterpret_cast<uint32_t *>(f.slice_off), 0u);
   }
   f.padded = padded;
   const size_t vbytes = val_type == SBLAS_VAL_F64 ? 8 : (val_type == SBLAS_VAL_F32 ? 4 : 2);
@@ -438,13 +550,37 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     return SBLAS_ERR_ALLOC;
   }
   {
+    // padding: value 0, column 0 (row order) or kSortedPad (sorted groups)
     const int64_t wv = static_cast<int64_t>(vbytes) * cap / 4, wc = cap * 2 / 4;
+    const uint32_t cpad = sorted ? (uint32_t{kSortedPad} << 16) | kSortedPad : 0u;
     hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((wv + 255) / 256)), dim3(256), 0, s,
-                       wv, static_cast<uint32_t *>(f.pval));
+                       wv, static_cast<uint32_t *>(f.pval), 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((wc + 255) / 256)), dim3(256), 0, s,
-                       wc, reinterpret_cast<uint32_t *>(f.pcol));
+                       wc, reinterpret_cast<uint32_t *>(f.pcol), cpad);
   }
-  if (f.npanels > 0) {
+  if (f.npanels > 0 && sorted) {
+    const dim3 g(f.npanels), b(256);
+    switch (val_type) {
+      case SBLAS_VAL_F64:
+        hipLaunchKernelGGL((k_panel_runs<double, true, true>), g, b, 0, s, f.npanels, R, f.info,
+                           win, rp, col, static_cast<const double *>(val), f.slice_base,
+                           f.slice_off, nullptr, nullptr, static_cast<double *>(f.pval), f.pcol,
+                           rank, f.goff);
+        break;
+      case SBLAS_VAL_F32:
+        hipLaunchKernelGGL((k_panel_runs<float, true, true>), g, b, 0, s, f.npanels, R, f.info,
+                           win, rp, col, static_cast<const float *>(val), f.slice_base,
+                           f.slice_off, nullptr, nullptr, static_cast<float *>(f.pval), f.pcol,
+                           rank, f.goff);
+        break;
+      default:
+        hipLaunchKernelGGL((k_panel_runs<uint16_t, true, true>), g, b, 0, s, f.npanels, R, f.info,
+                           win, rp, col, static_cast<const uint16_t *>(val), f.slice_base,
+                           f.slice_off, nullptr, nullptr, static_cast<uint16_t *>(f.pval), f.pcol,
+                           rank, f.goff);
+        break;
+    }
+  } else if (f.npanels > 0) {
     const dim3 g(f.npanels), b(256);
     switch (val_type) {
       case SBLAS_VAL_F64:
@@ -470,6 +606,7 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   release(cnt);
   release(win);
   release(q_n);
+  if (rank) release(rank);
   if (f.srp) {
     release(f.srp32);
     f.srp32 = nullptr;
@@ -521,6 +658,32 @@ int panel_launch_nx(const PanelFormat &f, int m, int n, int xcd_remap, const dou
                                                       beta, s);
 }
 
+template <typename VT, bool NX>
+int sorted_launch(const PanelFormat &f, int n, int xcd_remap, const double *x, double *y,
+                  double alpha, double beta, hipStream_t s) {
+  const int grid = xcd_remap ? ((f.npanels + 7) / 8) * 8 : f.npanels;
+#define SBLAS_SORTED_CASE(RPT)                                                                \
+  case RPT:                                                                                   \
+    hipLaunchKernelGGL((k_panel_sorted<VT, RPT, NX>), dim3(grid), dim3(256), 0, s, n, f.rows,  \
+                       f.npanels, xcd_remap, f.info, f.slice_base, f.slice_off, f.perm,       \
+                       f.goff, static_cast<const VT *>(f.pval), f.pcol, x, f.x32, y, alpha,   \
+                       beta, f.partials);                                                     \
+    break;
+  switch (f.rpt) {
+    SBLAS_SORTED_CASE(1)
+    SBLAS_SORTED_CASE(2)
+    SBLAS_SORTED_CASE(4)
+    SBLAS_SORTED_CASE(6)
+    SBLAS_SORTED_CASE(8)
+    default:
+      set_error(SBLAS_ERR_ARGS, "panel kernel: bad rows per thread");
+      return SBLAS_ERR_ARGS;
+  }
+#undef SBLAS_SORTED_CASE
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
 template <typename VT>
 int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const double *x, double *y,
                     double alpha, double beta, hipStream_t s) {
@@ -529,9 +692,13 @@ int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const dou
       // the FP32 copy of x first, on the same stream
       hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, f.x32);
       SBLAS_HIP_TRY(hipGetLastError());
+      if (f.layout == kPanelSorted)
+        return sorted_launch<VT, true>(f, n, xcd_remap, x, y, alpha, beta, s);
       return panel_launch_nx<VT, true>(f, m, n, xcd_remap, x, y, alpha, beta, s);
     }
   }
+  if (f.layout == kPanelSorted)
+    return sorted_launch<VT, false>(f, n, xcd_remap, x, y, alpha, beta, s);
   return panel_launch_nx<VT, false>(f, m, n, xcd_remap, x, y, alpha, beta, s);
 }
 
@@ -542,11 +709,20 @@ int panel_rpt(int rows) {
   return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : 8;
 }
 
-int panel_rows_for(int val_type, int m, int device, bool narrow_x) {
+int panel_rows_for(int val_type, int m, int device, bool narrow_x, int layout, int chunks) {
   int cus = 256, per_cu = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   const void *fn =
-      val_type == SBLAS_VAL_F64
+      layout == kPanelSorted
+          ? (val_type == SBLAS_VAL_F64
+                 ? reinterpret_cast<const void *>(&k_panel_sorted<double, 8, false>)
+             : val_type == SBLAS_VAL_F32
+                 ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_sorted<float, 8, true>)
+                             : reinterpret_cast<const void *>(&k_panel_sorted<float, 8, false>))
+                 : (narrow_x
+                        ? reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, 8, true>)
+                        : reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, 8, false>)))
+      : val_type == SBLAS_VAL_F64
           ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t, false>)
       : val_type == SBLAS_VAL_F32
           ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_spmv_nx<float, 8, uint16_t>)
@@ -558,6 +734,21 @@ int panel_rows_for(int val_type, int m, int device, bool narrow_x) {
       per_cu < 1)
     per_cu = 2;
   const int64_t slots = static_cast<int64_t>(std::max(cus, 1)) * per_cu;
+  if (layout == kPanelSorted) {
+    // one round of resident workgroups where the panels fit: a workgroup's
+    // slices follow one another, each waiting for its loads, so every
+    // panel in flight at once beats shorter panels in two rounds (FP32
+    // banded n = 2e6, 8 chunks: 1984 rows, one round, 0.344 ms; 1024 rows,
+    // 1.9 rounds, 0.414 ms).  Each chunk's last panel may be short: count
+    // one panel per chunk more.
+    for (int64_t k = 1;; k++) {
+      const int64_t room = k * slots - std::max(chunks, 1);
+      if (room < 1) continue;
+      int64_t R = (m + room - 1) / room;
+      R = (R + 63) / 64 * 64;
+      if (R <= kPanelRowsMax) return static_cast<int>(std::max<int64_t>(R, 64));
+    }
+  }
   // whole rounds of resident workgroups (at least two, so a slow workgroup is
   // not the whole tail), the tallest panels that fit
   for (int64_t k = 2;; k++) {

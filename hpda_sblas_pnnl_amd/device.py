@@ -36,7 +36,8 @@ def val_type_of(t) -> int:
 
 
 def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=0, sigma=0,
-              xcd_remap=-1, variant=0, items=0, panel_rows=0, narrow_x=0) -> Opts:
+              xcd_remap=-1, variant=0, items=0, panel_rows=0, narrow_x=0,
+              panel_layout=0) -> Opts:
     o = Opts()
     lib().sblas_opts_default(ctypes.byref(o))
     o.kernel = _KERNELS[kernel]
@@ -51,6 +52,7 @@ def make_opts(kernel="auto", chunks=0, part_mode="v1", nb=0, streams=1, subwave=
     o.items = items
     o.panel_rows = panel_rows
     o.narrow_x = narrow_x
+    o.panel_layout = panel_layout
     return o
 
 

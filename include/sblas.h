@@ -214,11 +214,17 @@ typedef struct sblas_opts {
   int panel_rows;  /* column-panel form: rows per row panel (1..2048, or   */
                    /* 1..1536 in a batch; 0 auto)                           */
   int narrow_x;    /* FP32 / BF16 values in the column-panel form: 0 auto   */
-                   /* (= on), 1 off: x staged in FP64, each row summed in   */
-                   /* the host loop's order; 2 on: x staged from an FP32    */
-                   /* copy, FP32 products summed in FP64 (error within a    */
-                   /* few FP32 roundings per product)                        */
-  int reserved[4];
+                   /* (on for a plan, off in a batch), 1 off: x staged in   */
+                   /* FP64, each row summed in the host loop's order; 2 on: */
+                   /* x staged from an FP32 copy, FP32 products summed in   */
+                   /* FP64 (error within a few FP32 roundings per product)  */
+  int panel_layout;  /* column-panel form of a plan, layout inside a slice: */
+                     /* 0 auto (2 where x is staged in FP32, else 1), 1    */
+                     /* rows in order (one thread per row sums staged      */
+                     /* products), 2 rows sorted by length in groups of 64 */
+                     /* (one lane per row, padded to the group's longest); */
+                     /* a batch uses 2 for narrow-x matrices, else 1       */
+  int reserved[3];
 } sblas_opts;
 
 /* y = alpha*A*x + beta*y for a CSR matrix on the device, merge path (kernel
@@ -282,6 +288,10 @@ typedef struct sblas_plan_info {
   float trial_ms[3]; /* trial medians in ms: row blocks (kernel 1 variant   */
                      /* 2), CSR5-style (kernel 3), column-sorted row blocks  */
                      /* (kernel 1 variant 4); 0 without a trial              */
+  int panel_layout;  /* column-panel form: 1 rows in order, 2 sorted groups */
+  int narrow_x;      /* column-panel form: 1 when x is staged in FP32       */
+  int64_t panel_elements; /* column-panel form: elements stored (padding    */
+                          /* included)                                      */
 } sblas_plan_info;
 int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info);
 /* Chunk descriptors of the plan (reference rule of opts.part_mode). */
@@ -341,8 +351,8 @@ int sblas_batch_run(sblas_batch batch, void *stream);
 /* The same work as one hipGraph (captured on first use, replayed after). */
 int sblas_batch_graph_launch(sblas_batch batch, void *stream);
 int sblas_batch_get_choice(sblas_batch batch, int index, sblas_batch_choice *out);
-/* work items, the larger persistent grid of the two launches, kernel
- * launches per run */
+/* work items, the largest persistent grid of the launches (sorted-layout row
+ * panels, row-order row panels, row blocks), kernel launches per run */
 int sblas_batch_get_shape(sblas_batch batch, int *work_items, int *grid, int *launches);
 int sblas_batch_destroy(sblas_batch batch);
 
