@@ -66,15 +66,14 @@ struct sblas_batch_s {
   int *d_qbound = nullptr;
   int *d_gmat = nullptr, *d_grow = nullptr, *d_gptr = nullptr, *d_gslot = nullptr;
   int groups = 0;
-  // FP32 copies of x for the narrow-x panel matrices, converted before the
-  // panel launch on every run
-  std::vector<sblas::X32Segment> x32segs;
-  sblas::X32Segment *d_x32segs = nullptr;
-  int64_t x32_longest = 0;
-  // item list: [0, n_narrow) row panels in the sorted-group layout (narrow-x
-  // matrices), then n_panel row panels in row order, then row blocks
-  int n_narrow = 0, n_panel = 0;
-  int grid_narrow = 0, grid_panel = 0, grid_rows = 0;
+  // narrow-x matrices in column-panel form (sorted-group layout, FP32 copy
+  // of x): each runs as its own launch of the plan's panel kernel, its row
+  // panels sized so that they are all resident at once
+  std::vector<int> solo;
+  // item list: [0, n_panel) row panels in row order, then row blocks
+  int n_panel = 0;
+  int grid_panel = 0, grid_rows = 0;
+  int device = 0;
   int items_per_thread = 8;
   std::vector<void *> bufs;
   hipStream_t cap_stream = nullptr;
@@ -114,34 +113,34 @@ struct sblas_batch_s {
     return SBLAS_OK;
   }
   int enqueue(hipStream_t s) {
-    // each item kind in its own persistent launch with its own queues
-    // (qbound) and counters (ctr): the sorted panels on s (after the FP32
-    // copies of x), the row-order panels and the row blocks on side streams
-    // forked from s and joined back before the merge, so the launches share
-    // the GPU instead of queueing
+    // the matrices with their own launches on s, one after another; the
+    // row-order panels and the row blocks each in one persistent launch with
+    // its own queues (qbound) and counters (ctr), on side streams forked
+    // from s and joined back before the merge, so the launches share the GPU
+    // instead of queueing
     constexpr int Q = sblas::kBatchQueues + 1;
-    const int n_rows = static_cast<int>(items.size()) - n_narrow - n_panel;
-    const int used = (grid_narrow > 0) + (grid_panel > 0) + (grid_rows > 0);
+    const int n_rows = static_cast<int>(items.size()) - n_panel;
+    const int used = (solo.empty() ? 0 : 1) + (grid_panel > 0) + (grid_rows > 0);
     // with one kind only, it runs on s itself
-    hipStream_t ps = (used > 1 && grid_narrow > 0) ? side2 : s;
+    hipStream_t ps = (used > 1 && !solo.empty()) ? side2 : s;
     hipStream_t rs = (used > 1) ? side : s;
     if (used > 1) {
       SBLAS_HIP_TRY(hipEventRecord(fork, s));
       if (grid_panel > 0 && ps != s) SBLAS_HIP_TRY(hipStreamWaitEvent(ps, fork, 0));
       if (grid_rows > 0 && rs != s) SBLAS_HIP_TRY(hipStreamWaitEvent(rs, fork, 0));
     }
-    int st = sblas::launch_batch_x32(static_cast<int>(x32segs.size()), d_x32segs, x32_longest, s);
-    if (st != SBLAS_OK) return st;
-    st = sblas::launch_batch(sblas::kBatchPanelNarrow, items_per_thread, grid_narrow, n_narrow,
-                             d_items, d_mats, d_partials, d_qbound, d_ctr, s);
-    if (st != SBLAS_OK) return st;
-    st = sblas::launch_batch(sblas::kBatchPanel, items_per_thread, grid_panel, n_panel,
-                             d_items + n_narrow, d_mats, d_partials, d_qbound + Q, d_ctr + Q, ps);
+    int st = sblas::launch_batch(sblas::kBatchPanel, items_per_thread, grid_panel, n_panel,
+                                 d_items, d_mats, d_partials, d_qbound, d_ctr, ps);
     if (st != SBLAS_OK) return st;
     st = sblas::launch_batch(sblas::kBatchRowBlock, items_per_thread, grid_rows, n_rows,
-                             d_items + n_narrow + n_panel, d_mats, d_partials, d_qbound + 2 * Q,
-                             d_ctr + 2 * Q, rs);
+                             d_items + n_panel, d_mats, d_partials, d_qbound + Q, d_ctr + Q, rs);
     if (st != SBLAS_OK) return st;
+    for (int i : solo) {
+      const sblas::BatchMatrix &M = mats[i];
+      st = sblas::launch_panel_spmv(M.val_type, panels[i], M.m, M.n, 1, M.x, M.y, M.alpha, M.beta,
+                                    s);
+      if (st != SBLAS_OK) return st;
+    }
     if (used > 1) {
       if (grid_panel > 0 && ps != s) {
         SBLAS_HIP_TRY(hipEventRecord(join2, ps));
@@ -173,6 +172,10 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     set_error(SBLAS_ERR_ALLOC, "batch: host allocation failed");
     return SBLAS_ERR_ALLOC;
   }
+  if (hipGetDevice(&b->device) != hipSuccess) b->device = 0;
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  cus = std::max(cus, 1);
   auto fail = [&](int code) {
     delete b;
     return code;
@@ -246,23 +249,51 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     b->panels.emplace_back();
     PanelFormat &f = b->panels.back();
     bool panel = false;
-    // narrow values in a batch: x staged in FP64 (rows in order, the host
-    // loop's order) unless asked for FP32 copies.  A batch runs each kind
-    // of item as one persistent launch, whose workgroups take many panels
-    // one after another: there the sorted-group body, which waits for its
-    // loads slice by slice, is slower (64-matrix FP32 batch: 5.84 ms against
-    // 4.67 ms staged in FP64).
+    // narrow values: x staged from an FP32 copy, the sorted-group layout and
+    // a launch of its own (the plan's kernel, row panels sized so that all
+    // are resident at once), unless asked not to.  In the persistent launch
+    // a workgroup takes many panels one after another, and the sorted body,
+    // which waits for its loads slice by slice, was slower there (64-matrix
+    // FP32 batch: 5.84 ms against 4.67 ms staged in FP64).
     int nxo = opts ? opts->narrow_x : 0;
-    if (nxo == 0) nxo = env_int_b("SBLAS_NARROW_X", 0) > 0 ? env_int_b("SBLAS_NARROW_X", 0) : 1;
-    const bool narrow = it.val_type != SBLAS_VAL_F64 && nxo == 2;
+    if (nxo == 0) nxo = env_int_b("SBLAS_NARROW_X", 0) > 0 ? env_int_b("SBLAS_NARROW_X", 0) : 2;
+    bool narrow = it.val_type != SBLAS_VAL_F64 && nxo == 2;
+    // a launch of its own only for a matrix whose row panels give every CU
+    // a workgroup (smaller ones left a few workgroups running for
+    // milliseconds); the others stage x in FP64 in the persistent launch
+    const int solo_rows =
+        narrow ? panel_rows_for(it.val_type, std::max(it.m, 1), b->device, true, kPanelSorted, 1)
+               : 0;
+    if (narrow && !(opts && opts->panel_rows > 0) &&
+        (static_cast<int64_t>(it.m) + solo_rows - 1) / solo_rows < cus)
+      narrow = false;
     if (kv == 1 && it.m > 0 && it.nnz > 0 && want != 1 && want != 2 &&
         (want == 3 || rs.max_row <= 8192)) {
       const std::vector<ExecChunk> one{ExecChunk{0, it.nnz, 0, it.m}};
-      const int rows_max = narrow ? kBatchPanelRowsNarrow : kBatchPanelRowsMax;
-      int rows = (opts && opts->panel_rows > 0) ? std::min(opts->panel_rows, rows_max) : rows_max;
-      const int layout = narrow ? kPanelSorted : kPanelRowOrder;
+      int rows = (opts && opts->panel_rows > 0) ? std::min(opts->panel_rows, kBatchPanelRowsMax)
+                 : narrow ? solo_rows
+                          : kBatchPanelRowsMax;
+      int layout = narrow ? kPanelSorted : kPanelRowOrder;
       int st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col,
                            it.d_val, one, h_rp.data(), want == 3, 1, f, &panel, nullptr, layout);
+      // panels sized for one round may hold slices too sparse for the form:
+      // taller ones while every CU still gets one
+      while (st == SBLAS_OK && !panel && narrow && rows < 1024 && !(opts && opts->panel_rows > 0) &&
+             (static_cast<int64_t>(it.m) + 2 * rows - 1) / (2 * rows) >= cus) {
+        f = PanelFormat{};
+        rows = std::min(2 * rows, 1024);
+        st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col, it.d_val,
+                         one, h_rp.data(), want == 3, 1, f, &panel, nullptr, layout);
+      }
+      if (st == SBLAS_OK && !panel && narrow && !(opts && opts->panel_rows > 0)) {
+        // no launch of its own: the persistent launch's row panels, x in FP64
+        narrow = false;
+        f = PanelFormat{};
+        rows = kBatchPanelRowsMax;
+        layout = kPanelRowOrder;
+        st = panel_build(it.val_type, rows, it.m, it.n, it.nnz, it.d_row_ptr, it.d_col, it.d_val,
+                         one, h_rp.data(), want == 3, 1, f, &panel, nullptr, layout);
+      }
       // row order with slices beyond 16-bit offsets (the batch kernel reads
       // 16-bit ones only): shorter row panels, down to 192 rows, before
       // giving up (the sorted layout has no such limit)
@@ -295,10 +326,11 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       M.pval = f.pval;
       M.pcol = f.pcol;
       M.prows = f.rows;
-      M.perm = f.perm;
-      M.goff = f.goff;
       c.chunks = 1;
       if (narrow) {
+        // FP32 copy of x, zero-padded to whole column panels, written by
+        // the launch itself
+        b->solo.push_back(i);
         const int64_t padded = (static_cast<int64_t>(it.n) + kPanelCols - 1) / kPanelCols *
                                kPanelCols;
         void *p = nullptr;
@@ -311,9 +343,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
           set_error(SBLAS_ERR_HIP, "batch: FP32 x copy set-up failed");
           return fail(SBLAS_ERR_HIP);
         }
-        M.x32 = static_cast<float *>(p);
-        b->x32segs.push_back(X32Segment{it.d_x, static_cast<float *>(p), it.n});
-        b->x32_longest = std::max<int64_t>(b->x32_longest, it.n);
+        f.x32 = static_cast<float *>(p);
       }
     }
     b->mats.push_back(M);
@@ -323,11 +353,12 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       continue;
     }
     std::vector<Pending> &mine = pending[i];
-    if (panel) {
+    if (panel && narrow) {
+      // its own launch: no items
+    } else if (panel) {
       for (size_t q = 0; q < f.hpanels.size(); q++) {
         const ExecChunk &e = f.hpanels[q];
-        mine.push_back(Pending{BatchItem{i, narrow ? kBatchPanelNarrow : kBatchPanel,
-                                         static_cast<int>(q), 0, 0, 0, 0, 0},
+        mine.push_back(Pending{BatchItem{i, kBatchPanel, static_cast<int>(q), 0, 0, 0, 0, 0},
                                e.hi - e.lo, static_cast<int>(e.lo), static_cast<int>(e.hi),
                                e.row_begin, e.row_end});
       }
@@ -341,16 +372,18 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
                                r.hi - r.lo, r.lo, r.hi, r.row_begin, r.row_end});
     }
     b->choices.push_back(sblas_batch_choice{SBLAS_KERNEL_VECTOR, c.subwave, c.items, c.chunks,
-                                            static_cast<int>(mine.size()), panel ? 3 : 2, rs.mean,
+                                            static_cast<int>(panel && narrow ? f.npanels
+                                                                             : mine.size()),
+                                            panel ? 3 : 2, rs.mean,
                                             rs.cv, rs.max_row, panel ? f.rows : 0, 0});
     h_rps[i] = std::move(h_rp);
   }
-  // list order: the sorted-layout row panels, the row-order row panels, then
-  // the row-block items (one persistent launch each); within a kind matrix by
-  // matrix, each matrix's items in its own order (row panels of one matrix
-  // stay neighbours, so they share x in one XCD's L2)
+  // list order: the row-panel items, then the row-block items (one
+  // persistent launch each); within a kind matrix by matrix, each matrix's
+  // items in its own order (row panels of one matrix stay neighbours, so
+  // they share x in one XCD's L2)
   std::vector<std::pair<int, int>> order;  // (matrix, index in its list)
-  for (int kind : {kBatchPanelNarrow, kBatchPanel, kBatchRowBlock})
+  for (int kind : {kBatchPanel, kBatchRowBlock})
     for (int i = 0; i < count; i++)
       for (int q = 0; q < static_cast<int>(pending[i].size()); q++)
         if (pending[i][q].item.kind == kind) order.emplace_back(i, q);
@@ -371,15 +404,12 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
   }
   // per kind, eight queues of about equal work, contiguous in that kind's
   // part of the list (bounds relative to its first item)
-  size_t n_narrow = 0, n_panel = 0;
-  while (n_narrow < b->items.size() && b->items[n_narrow].kind == kBatchPanelNarrow) n_narrow++;
-  while (n_narrow + n_panel < b->items.size() && b->items[n_narrow + n_panel].kind == kBatchPanel)
-    n_panel++;
-  b->n_narrow = static_cast<int>(n_narrow);
+  size_t n_panel = 0;
+  while (n_panel < b->items.size() && b->items[n_panel].kind == kBatchPanel) n_panel++;
   b->n_panel = static_cast<int>(n_panel);
   std::vector<int> qbound;
-  const size_t cut[4] = {0, n_narrow, n_narrow + n_panel, order.size()};
-  for (int part = 0; part < 3; part++) {
+  const size_t cut[3] = {0, n_panel, order.size()};
+  for (int part = 0; part < 2; part++) {
     const size_t w0 = cut[part], w1 = cut[part + 1];
     auto work = [&](size_t w) {  // elements + per-item cost
       return pending[order[w].first][order[w].second].work * b->mats[order[w].first].k + 256;
@@ -427,7 +457,6 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
   int st;
   if ((st = b->upload(&b->d_mats, b->mats)) != SBLAS_OK ||
       (st = b->upload(&b->d_items, b->items)) != SBLAS_OK ||
-      (st = b->upload(&b->d_x32segs, b->x32segs)) != SBLAS_OK ||
       (st = b->upload(&b->d_gmat, gmat)) != SBLAS_OK ||
       (st = b->upload(&b->d_grow, grow)) != SBLAS_OK ||
       (st = b->upload(&b->d_gptr, gptr)) != SBLAS_OK ||
@@ -435,25 +464,18 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
     return fail(st);
   std::vector<double> zeros(static_cast<size_t>(std::max<int64_t>(nslots, 1)), 0.0);
   if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);
-  if ((st = b->upload(&b->d_ctr, std::vector<int>(3 * (kBatchQueues + 1), 0))) != SBLAS_OK ||
+  if ((st = b->upload(&b->d_ctr, std::vector<int>(2 * (kBatchQueues + 1), 0))) != SBLAS_OK ||
       (st = b->upload(&b->d_qbound, qbound)) != SBLAS_OK)
     return fail(st);
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess)
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   // per kind the resident workgroups, never more than there are items.  With
   // another kind beside them, panels take one workgroup per CU less than fit:
   // the panel workgroups fill a CU's LDS, and the other launch would wait for
   // the panel launch to finish (64-matrix batch: 4.87 -> 4.69 ms)
-  const int n_rows = static_cast<int>(b->items.size()) - b->n_narrow - b->n_panel;
-  const int kinds = (b->n_narrow > 0) + (b->n_panel > 0) + (n_rows > 0);
-  auto panel_grid = [&](int kind, int nitems) {
-    const int occ = batch_occupancy(kind, b->items_per_thread);
-    const int per_cu = (kinds > 1 && occ > 1) ? occ - 1 : occ;
-    return std::min<int>(nitems, std::max(1, cus) * per_cu);
-  };
-  b->grid_narrow = panel_grid(kBatchPanelNarrow, b->n_narrow);
-  b->grid_panel = panel_grid(kBatchPanel, b->n_panel);
+  const int n_rows = static_cast<int>(b->items.size()) - b->n_panel;
+  const int kinds = (b->solo.empty() ? 0 : 1) + (b->n_panel > 0) + (n_rows > 0);
+  const int occ_panel = batch_occupancy(kBatchPanel, b->items_per_thread);
+  const int panel_per_cu = (kinds > 1 && occ_panel > 1) ? occ_panel - 1 : occ_panel;
+  b->grid_panel = std::min<int>(b->n_panel, std::max(1, cus) * panel_per_cu);
   b->grid_rows = std::min<int>(n_rows, std::max(1, cus) *
                                            batch_occupancy(kBatchRowBlock, b->items_per_thread));
   if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
@@ -509,10 +531,10 @@ int sblas_batch_get_choice(sblas_batch b, int index, sblas_batch_choice *out) {
 int sblas_batch_get_shape(sblas_batch b, int *work_items, int *grid, int *launches) {
   if (!b) return SBLAS_ERR_ARGS;
   if (work_items) *work_items = static_cast<int>(b->items.size());
-  if (grid) *grid = std::max({b->grid_narrow, b->grid_panel, b->grid_rows});
+  if (grid) *grid = std::max(b->grid_panel, b->grid_rows);
   if (launches)
-    *launches = (b->grid_narrow > 0 ? 1 : 0) + (b->grid_panel > 0 ? 1 : 0) +
-                (b->grid_rows > 0 ? 1 : 0) + (b->groups > 0 ? 1 : 0) + (b->x32segs.empty() ? 0 : 1);
+    *launches = (b->grid_panel > 0 ? 1 : 0) + (b->grid_rows > 0 ? 1 : 0) +
+                (b->groups > 0 ? 1 : 0) + 2 * static_cast<int>(b->solo.size());
   return SBLAS_OK;
 }
 

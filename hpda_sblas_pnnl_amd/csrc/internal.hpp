@@ -255,30 +255,9 @@ struct BatchMatrix {
   const uint16_t *pcol;
   int prows;    // rows per row panel
   int k;        // right-hand sides (x, y are n x k, m x k row-major); 1 = one vector
-  // narrow-x column panels (FP32 / BF16 values): FP32 copy of x, padded with
-  // zeros to whole column panels, written before the panel launch; nullptr:
-  // x staged in FP64
-  const float *x32;
-  // sorted-group slice layout (the narrow-x panels): row order and group
-  // offsets of each slice; nullptr: rows in order (srp)
-  const uint16_t *perm;
-  const uint32_t *goff;
 };
-// One matrix's x to convert for narrow-x panels: x32[0, n) = x[0, n) in FP32
-// (one grid row of the conversion launch per segment; longest = the largest n).
-struct X32Segment {
-  const double *x;
-  float *x32;
-  int64_t n;
-};
-int launch_batch_x32(int nseg, const X32Segment *segs, int64_t longest, hipStream_t s);
 constexpr int kBatchRowBlock = 0;
 constexpr int kBatchPanel = 1;
-// launch kind for a batch whose row panels all stage x in FP32 (stored in
-// the sorted-group layout): a kernel with that body only, at most
-// kBatchPanelRowsNarrow rows per panel
-constexpr int kBatchPanelNarrow = 2;
-constexpr int kBatchPanelRowsNarrow = 1024;
 struct BatchItem {  // one row block, or one row panel, of one matrix
   int mat;
   int kind;     // kBatchRowBlock or kBatchPanel
@@ -287,8 +266,8 @@ struct BatchItem {  // one row block, or one row panel, of one matrix
   int pslot;               // first partial slot: 2*k slots (first row, last row)
   int pad_;
 };
-// Persistent launch over the items of one kind (kBatchPanel, kBatchPanelNarrow
-// or kBatchRowBlock): the list is cut into kBatchQueues contiguous queues
+// Persistent launch over the items of one kind (kBatchPanel or
+// kBatchRowBlock): the list is cut into kBatchQueues contiguous queues
 // [qbound[q], qbound[q+1]); workgroups take items from per-queue counters
 // ctr[q] (ctr[kBatchQueues] counts finished workgroups; the last one resets
 // them all).  Each item's arithmetic does not depend on which workgroup runs it.

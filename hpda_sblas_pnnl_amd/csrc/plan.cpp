@@ -799,6 +799,15 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
     bool chosen = false;
     st = panel_build(val_type, rows, m, n, nnz, a0, a1, val, exec, h_rp.data(),
                      opts.variant == 3, min_panels, panel, &chosen, nullptr, layout);
+    // sorted-layout panels sized for one round may hold slices too sparse
+    // for the form: taller ones (while enough panels remain) before giving up
+    for (int r2 = 2 * rows; st == SBLAS_OK && !chosen && layout == kPanelSorted &&
+                            opts.panel_rows <= 0 && r2 <= 1024 && m / r2 >= min_panels;
+         r2 *= 2) {
+      panel = PanelFormat{};
+      st = panel_build(val_type, r2, m, n, nnz, a0, a1, val, exec, h_rp.data(), false, min_panels,
+                       panel, &chosen, nullptr, layout);
+    }
     if (st == SBLAS_ERR_UNSUPPORTED && opts.variant == 0) {
       // a slice too large for 16-bit offsets: keep the row-block form
       for (void *p : panel.buffers) (void)hipFree(p);

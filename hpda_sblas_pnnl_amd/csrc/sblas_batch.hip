@@ -14,7 +14,6 @@
 // partials that one merge launch combines in item order.
 #include "internal.hpp"
 #include "panel_device.hpp"
-#include "sorted_panel_device.hpp"
 
 namespace sblas {
 namespace {
@@ -231,24 +230,6 @@ __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchM
   }
 }
 
-// One row panel of a matrix whose x is staged from its FP32 copy (FP32 or
-// BF16 values) in the sorted-group layout, at most kBatchPanelRowsNarrow rows.
-__device__ __forceinline__ void row_panel_narrow(const BatchItem &it, const BatchMatrix &M,
-                                                 double *__restrict__ partials, double *lds) {
-  constexpr int RPT = kBatchPanelRowsNarrow / 256;
-  const int p = it.row_begin;
-  const PanelInfo pi = M.pinfo[p];
-  double *slots = partials + it.pslot;
-  if (M.val_type == SBLAS_VAL_F32)
-    sorted_panel::panel<float, RPT, true>(p, M.n, M.prows, pi, M.slice_base, M.slice_off, M.perm,
-                                          M.goff, static_cast<const float *>(M.pval), M.pcol,
-                                          M.x, M.x32, M.y, M.alpha, M.beta, slots, lds);
-  else
-    sorted_panel::panel<uint16_t, RPT, true>(p, M.n, M.prows, pi, M.slice_base, M.slice_off,
-                                             M.perm, M.goff, static_cast<const uint16_t *>(M.pval),
-                                             M.pcol, M.x, M.x32, M.y, M.alpha, M.beta, slots, lds);
-}
-
 // One row panel of a matrix in column-panel form (16-bit slice offsets).
 __device__ __forceinline__ void row_panel_any(const BatchItem &it, const BatchMatrix &M,
                                               double *__restrict__ partials, double *lds) {
@@ -289,11 +270,10 @@ __device__ __forceinline__ void rows_typed(int w, const BatchItem &it, const Bat
   }
 }
 
-// The persistent loop of one item kind (MODE kBatchPanel: row panels,
-// kBatchPanelNarrow: row panels that all stage x in FP32, else row blocks).
-// Each kind is its own kernel compiled with only its own body, so its
-// registers and LDS (52 KB for panels, 36 KB for narrow panels, 20-36 KB for
-// row blocks) set its own occupancy.
+// The persistent loop of one item kind (MODE kBatchPanel: row panels, else
+// row blocks).  Each kind is its own kernel compiled with only its own body,
+// so its registers and LDS (52 KB for panels, 20-36 KB for row blocks) set
+// its own occupancy.
 template <int MODE, int ITEMS, int BLOCK>
 __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
                                            const BatchMatrix *__restrict__ mats,
@@ -301,11 +281,9 @@ __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
                                            const int *__restrict__ qbound,
                                            int *__restrict__ ctr) {
   static_assert(BLOCK == 256, "the panel body assumes 256 threads");
-  constexpr int LDS =
-      MODE == kBatchPanel ? panel::lds_doubles<uint16_t, kBatchPanelRowsMax / 256>()
-      : MODE == kBatchPanelNarrow
-          ? sorted_panel::lds_doubles<kBatchPanelRowsNarrow / 256, true>()
-          : BLOCK * ITEMS + kStreamRowsMax;
+  constexpr int LDS = MODE == kBatchPanel
+                          ? panel::lds_doubles<uint16_t, kBatchPanelRowsMax / 256>()
+                          : BLOCK * ITEMS + kStreamRowsMax;
   __shared__ double lds[LDS];
   __shared__ double wsum[BLOCK / 64];
   __shared__ int next;
@@ -330,8 +308,6 @@ __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
       const BatchMatrix &M = mats[it.mat];
       if constexpr (MODE == kBatchPanel) {
         row_panel_any(it, M, partials, lds);
-      } else if constexpr (MODE == kBatchPanelNarrow) {
-        row_panel_narrow(it, M, partials, lds);
       } else if (M.val_type == SBLAS_VAL_F64) {  // value type chosen once per item
         rows_typed<double, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
       } else if (M.val_type == SBLAS_VAL_F32) {
@@ -363,26 +339,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
   (void)nitems;
 }
 
-// Narrow panels only: at most 128 registers a lane, so four workgroups share
-// a CU (their 36 KB of LDS each fit too).
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_nx(
-    int nitems, const BatchItem *__restrict__ items, const BatchMatrix *__restrict__ mats,
-    double *__restrict__ partials, const int *__restrict__ qbound, int *__restrict__ ctr) {
-  batch_loop<kBatchPanelNarrow, 8, BLOCK>(items, mats, partials, qbound, ctr);
-  (void)nitems;
-}
-
-// FP32 copies of the narrow-x panel matrices' x: row blockIdx.y of the grid
-// converts segment blockIdx.y, its workgroups striding over the elements.
-__global__ void k_batch_x32(const X32Segment *__restrict__ segs) {
-  const X32Segment sg = segs[blockIdx.y];  // segs starts at this launch's first segment
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < sg.n;
-       j += stride)
-    sg.x32[j] = static_cast<float>(sg.x[j]);
-}
-
 // Split rows of all matrices: one thread per row (all its vectors), partials
 // in item order.
 __global__ void k_batch_merge(int groups, const int *__restrict__ gmat,
@@ -412,9 +368,6 @@ int launch_batch(int kind, int items_per_thread, int grid, int nitems, const Bat
   if (kind == kBatchPanel)
     hipLaunchKernelGGL((k_batch<P, 8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats,
                        partials, qbound, ctr);
-  else if (kind == kBatchPanelNarrow)
-    hipLaunchKernelGGL((k_batch_nx<B>), dim3(grid), dim3(B), 0, s, nitems, items, mats, partials,
-                       qbound, ctr);
   else if (items_per_thread == 16)
     hipLaunchKernelGGL((k_batch<R, 16, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats,
                        partials, qbound, ctr);
@@ -429,27 +382,13 @@ int batch_occupancy(int kind, int items_per_thread) {
   int per_cu = 0;
   constexpr int P = kBatchPanel, R = kBatchRowBlock;
   const void *fn =
-      kind == kBatchPanel        ? reinterpret_cast<const void *>(&k_batch<P, 8, kStreamBlock>)
-      : kind == kBatchPanelNarrow ? reinterpret_cast<const void *>(&k_batch_nx<kStreamBlock>)
+      kind == kBatchPanel      ? reinterpret_cast<const void *>(&k_batch<P, 8, kStreamBlock>)
       : items_per_thread == 16 ? reinterpret_cast<const void *>(&k_batch<R, 16, kStreamBlock>)
                                : reinterpret_cast<const void *>(&k_batch<R, 8, kStreamBlock>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kStreamBlock, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   return per_cu;
-}
-
-int launch_batch_x32(int nseg, const X32Segment *segs, int64_t longest, hipStream_t s) {
-  if (nseg <= 0 || longest <= 0) return SBLAS_OK;
-  // up to 256 workgroups per segment, each thread about 8 elements of the longest
-  const int64_t per = std::min<int64_t>(256, (longest + 2047) / 2048);
-  for (int s0 = 0; s0 < nseg; s0 += 65535) {  // grid rows per launch
-    const int cnt = std::min(nseg - s0, 65535);
-    hipLaunchKernelGGL(k_batch_x32, dim3(static_cast<unsigned>(per), static_cast<unsigned>(cnt)),
-                       dim3(256), 0, s, segs + s0);
-  }
-  SBLAS_HIP_TRY(hipGetLastError());
-  return SBLAS_OK;
 }
 
 int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,

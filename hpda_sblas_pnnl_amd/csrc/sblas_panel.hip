@@ -709,6 +709,35 @@ int panel_rpt(int rows) {
   return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : 8;
 }
 
+namespace {
+// resident workgroups per CU of the sorted-group kernel with RPT rows per thread
+template <int RPT>
+int sorted_occupancy_rpt(int val_type, bool narrow_x) {
+  const void *fn =
+      val_type == SBLAS_VAL_F64
+          ? reinterpret_cast<const void *>(&k_panel_sorted<double, RPT, false>)
+      : val_type == SBLAS_VAL_F32
+          ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_sorted<float, RPT, true>)
+                      : reinterpret_cast<const void *>(&k_panel_sorted<float, RPT, false>))
+          : (narrow_x ? reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, RPT, true>)
+                      : reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, RPT, false>));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return per_cu;
+}
+int sorted_occupancy(int val_type, bool narrow_x, int rpt) {
+  switch (rpt) {
+    case 1: return sorted_occupancy_rpt<1>(val_type, narrow_x);
+    case 2: return sorted_occupancy_rpt<2>(val_type, narrow_x);
+    case 4: return sorted_occupancy_rpt<4>(val_type, narrow_x);
+    case 6: return sorted_occupancy_rpt<6>(val_type, narrow_x);
+    default: return sorted_occupancy_rpt<8>(val_type, narrow_x);
+  }
+}
+}  // namespace
+
 int panel_rows_for(int val_type, int m, int device, bool narrow_x, int layout, int chunks) {
   int cus = 256, per_cu = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
@@ -739,14 +768,19 @@ int panel_rows_for(int val_type, int m, int device, bool narrow_x, int layout, i
     // slices follow one another, each waiting for its loads, so every
     // panel in flight at once beats shorter panels in two rounds (FP32
     // banded n = 2e6, 8 chunks: 1984 rows, one round, 0.344 ms; 1024 rows,
-    // 1.9 rounds, 0.414 ms).  Each chunk's last panel may be short: count
-    // one panel per chunk more.
+    // 1.9 rounds, 0.414 ms).  The fewest rows per thread (the most
+    // workgroups per CU) whose kernel holds every panel at once; each
+    // chunk's last panel may be short, so one panel per chunk more.
     for (int64_t k = 1;; k++) {
-      const int64_t room = k * slots - std::max(chunks, 1);
-      if (room < 1) continue;
-      int64_t R = (m + room - 1) / room;
-      R = (R + 63) / 64 * 64;
-      if (R <= kPanelRowsMax) return static_cast<int>(std::max<int64_t>(R, 64));
+      for (const int rpt : {1, 2, 4, 6, 8}) {
+        const int occ = sorted_occupancy(val_type, narrow_x, rpt);
+        const int64_t room = k * static_cast<int64_t>(std::max(cus, 1)) * occ -
+                             std::max(chunks, 1);
+        if (room < 1) continue;
+        int64_t R = (m + room - 1) / room;
+        R = (R + 63) / 64 * 64;
+        if (R <= 256 * rpt) return static_cast<int>(std::max<int64_t>(R, 64));
+      }
     }
   }
   // whole rounds of resident workgroups (at least two, so a slow workgroup is

@@ -41,10 +41,14 @@ __host__ __device__ constexpr int lds_doubles() {
   return (NX ? kPanelCols / 2 : kPanelCols) + ((RPT + 1) / 2) * 128 + 2 * RPT + 1 + 256 * RPT;
 }
 // Elements of each of its rows a lane loads per round: the rounds of a slice
-// follow one another (each waits for its loads), so more per round means
-// fewer waits; three or four keep a lane's registers within a quarter of
-// the file (four workgroups per CU).
-__host__ __device__ constexpr int unroll_for(int rpt) { return rpt >= 6 ? 3 : 4; }
+// follow one another (each waits for its loads), so a lane keeps about 16
+// loads in flight whatever its rows per thread (rows of many elements per
+// slice need them: at one row per lane and 4 per round, 46 elements a row
+// per slice ran 0.16 ms where rows in order took 0.10), and at 6 or 8 rows
+// three per row keep its registers within a quarter of the file.
+__host__ __device__ constexpr int unroll_for(int rpt) {
+  return rpt <= 1 ? 16 : rpt == 2 ? 8 : rpt <= 4 ? 4 : 3;
+}
 
 template <typename VT>
 __device__ __forceinline__ double widen1(VT v) {

@@ -147,10 +147,9 @@ def test_batch_panels(oracle, variant, panel_rows):
 
 def test_batch_three_kinds(oracle):
     """With narrow_x = 2, FP32 / BF16 matrices in column-panel form take the
-    sorted-group layout with x from FP32 copies (their own launch), FP64 ones
-    rows in order
-    (another launch), R-MAT matrices row blocks (a third), joined before the
-    merge: FP64 panel rows bitwise equal to the oracle, row blocks within
+    sorted-group layout with x from FP32 copies (a launch per matrix), FP64
+    ones rows in order (a persistent launch), R-MAT matrices row blocks
+    (another), joined before the merge: FP64 panel rows bitwise equal to the oracle, row blocks within
     1e-12, narrow panel rows within two FP32 roundings per product; eager
     runs and graph replays bitwise equal.  One narrow matrix has an odd n
     (its last column in use) with x 8 bytes past a 16-byte boundary."""
@@ -196,7 +195,8 @@ def test_batch_three_kinds(oracle):
     kinds = {name: c["variant"] for (name, _, _), c in zip(mats, ch)}
     assert kinds["wide64"] == 3 and kinds["wide32"] == 3 and kinds["widebf"] == 3
     assert kinds["odd32"] == 3 and kinds["rmat"] == 2 and kinds["rmat64"] == 2
-    assert batch.shape()["launches"] >= 4  # x copies, two panel launches, row blocks
+    # an x copy and a panel launch per narrow matrix, row-order panels, row blocks
+    assert batch.shape()["launches"] >= 2 * 3 + 2
 
     def check(outs):
         for (name, A, vt), got, ref, vr, x, y0, c, it in zip(mats, outs, refs, vrs, xs, y0s, ch,
@@ -307,11 +307,12 @@ def test_batch_shorter_row_panels(oracle):
 
 @pytest.mark.parametrize("vt", ["f32", "bf16"])
 def test_batch_narrow_x_panels(oracle, vt):
-    """FP32 / BF16 matrices in column-panel form with narrow_x = 2 stage x
-    from FP32 copies made inside the launch (sorted-group layout): every row
-    within two FP32 roundings per product of the oracle on the same rounded
-    values, eager and graph replays bitwise equal; narrow_x = 1 (the batch
-    default) keeps the host loop's order (bitwise equal to that oracle)."""
+    """FP32 / BF16 matrices in column-panel form with narrow_x = 2 (the
+    default) stage x from FP32 copies (sorted-group layout, a launch per
+    matrix): every row within two FP32 roundings per product of the oracle on
+    the same rounded values, eager and graph replays bitwise equal;
+    narrow_x = 1 keeps the host loop's order (bitwise equal to that
+    oracle)."""
     import torch
     from hpda_sblas_pnnl_amd import gen as G
     from hpda_sblas_pnnl_amd.device import SpmvBatch

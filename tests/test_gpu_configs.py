@@ -308,9 +308,8 @@ def test_batch64_fp32_graph(torch_dev, oracle):
         outs.append([it["y"].cpu().numpy() for it in items])
     for i, (kind, A) in enumerate(mats):
         if batch.choice(i)["variant"] == 3:
-            # column panels (x staged in FP64 by default in a batch): each
-            # row within two FP32 roundings per product of the FP32-value
-            # oracle however x is staged
+            # column panels stage x in FP32: each row within two FP32
+            # roundings per product of the FP32-value oracle
             vr = A.val.astype(np.float32).astype(np.float64)
             x = G.values(A.n, i, 2, "uniform")
             beta = 0.5 if i % 3 == 0 else 0.0
