@@ -745,9 +745,11 @@ int Plan::trial(const std::vector<std::pair<int, int>> &forms, std::vector<float
 // With kernel AUTO the autotuner decides from the matrix's structure and row
 // statistics: the column-panel form of kernel 1 where the column structure
 // pays for it (a structural test over the per-panel column ranges, measured
-// crossover); otherwise the row-block form of kernel 1 or the CSR5-style
-// kernel 3 (tile depth sigma from the reference's rule), whichever a timed
-// trial on this matrix finds faster.  Trials run for matrices of at least
+// crossover; slices in row order for FP64 values, in sorted groups where x
+// is staged in FP32); otherwise the row-block form of kernel 1, its
+// column-sorted variant or the CSR5-style kernel 3 (tile depth sigma from the
+// reference's rule), whichever a timed trial on this matrix finds fastest.
+// Trials run for matrices of at least
 // 2^20 elements (smaller ones take row blocks) and are cached by row profile
 // (value type, mean, coefficient of variation, empty-row fraction, size),
 // so later plans of the same profile reuse the decision without a trial.
