@@ -509,6 +509,30 @@ def extras_powerlaw(log_to):
            "nnz": A.nnz, "ms": round(ms, 4), "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
            "err_normwise": err, "kernel": inf["kernel"], "variant": inf["variant"],
            "cpu_baseline": cpu_sample(A, x, y0, 1.0, 0.0, reps=50)}
+    # narrow values (FP64 x, y and sums) and k = 8 right-hand sides on the same matrix
+    prec = {}
+    for name, vals in (("fp32_values", A.val.astype(np.float32)),
+                       ("bf16_values", G.to_bf16_bits(A.val).view(np.int16))):
+        pv = SpmvPlan("csr", A.m, A.n, rp, col, torch.from_numpy(vals).to(dev))
+        pv.spmv(dx, dy, 1.0, 0.0)
+        torch.cuda.synchronize()
+        e = ora.err_normwise(dy.cpu().numpy(), ref)
+        msv = _time_calls(lambda: pv.spmv(dx, dy, 1.0, 0.0), reps=200)
+        prec[name] = {"ms": round(msv, 4), "speedup_vs_fp64": round(ms / msv, 3),
+                      "err_normwise": e, "variant": pv.info()["variant"]}
+    k = 8
+    X = G.values(A.n * k, 0, 4, "uniform").reshape(A.n, k)
+    dX = torch.from_numpy(X).to(dev)
+    dY = torch.zeros((A.m, k), dtype=torch.float64, device=dev)
+    p.spmm(dX, dY, 1.0, 0.0)
+    torch.cuda.synchronize()
+    refk = ora.csr_spmm(A.m, k, A.val, A.row_ptr, A.col, X, 1.0, 0.0, np.zeros((A.m, k)))
+    ek = ora.err_normwise(dY.cpu().numpy().ravel(), np.asarray(refk).ravel())
+    msk = _time_calls(lambda: p.spmm(dX, dY, 1.0, 0.0), reps=200)
+    prec["spmm_k8_fp64"] = {"ms": round(msk, 4), "ms_per_vector": round(msk / k, 4),
+                            "speedup_per_vector_vs_fp64": round(ms * k / msk, 3),
+                            "err_normwise": ek}
+    out["precision"] = prec
     log_to(f"[bench] powerlaw: {out}")
     return out
 

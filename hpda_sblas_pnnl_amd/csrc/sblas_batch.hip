@@ -215,16 +215,48 @@ __device__ __forceinline__ void row_block_mrhs(const BatchItem &it, const BatchM
   const int sub = M.subwave < K ? K : M.subwave;  // power of two in [K, 64]
   const int subs = BLOCK / sub;
   const int grp = (t % sub) / K;
+  // a row of more than 16 steps of its lane groups is left to whole waves
+  // afterwards (64/K groups per vector), as in the plans' multi-RHS kernel
+  const int longest = 16 * (sub / K);
+  auto finish = [&](int r, int rs, int re, double sum) {
+    if (rs < lo || re > hi) {
+      partials[it.pslot + (r == it.row_begin ? 0 : K) + vec] = sum;
+    } else {
+      double *yp = Y + static_cast<int64_t>(r) * K + vec;
+      *yp = b_axpby(M.alpha, sum, M.beta, *yp);
+    }
+  };
   for (int r = it.row_begin + t / sub; r < it.row_end; r += subs) {
     const int rs = rp[r], re = rp[r + 1];
+    if (sub < 64 && min(re, hi) - max(rs, lo) > longest) continue;
     double sum = dot(max(rs, lo) + grp, min(re, hi), sub / K);
     for (int off = sub >> 1; off >= K; off >>= 1) sum += __shfl_xor(sum, off, 64);
-    if (grp == 0) {
-      if (rs < lo || re > hi) {
-        partials[it.pslot + (r == it.row_begin ? 0 : K) + vec] = sum;
-      } else {
-        double *yp = Y + static_cast<int64_t>(r) * K + vec;
-        *yp = b_axpby(M.alpha, sum, M.beta, *yp);
+    if (grp == 0) finish(r, rs, re, sum);
+  }
+  if (sub < 64) {
+    // long rows in block order, found by a ballot (the same list in every
+    // wave) and dealt out to the waves in turn
+    const int w = t >> 6, l = t & 63, g64 = l / K;
+    int seen = 0;
+    for (int base = 0; base < nrows; base += 64) {
+      bool is_long = false;
+      if (base + l < nrows) {
+        const int r = it.row_begin + base + l;
+        is_long = min(rp[r + 1], hi) - max(rp[r], lo) > longest;
+      }
+      uint64_t mask = __ballot(is_long);
+      while (mask) {
+        const int bit = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        mask &= mask - 1;
+        if (seen % (BLOCK / 64) == w) {
+          const int r = it.row_begin + base + bit;
+          const int rs = rp[r], re = rp[r + 1];
+          double sum = dot(max(rs, lo) + g64, min(re, hi), 64 / K);
+#pragma unroll
+          for (int off = 32; off >= K; off >>= 1) sum += __shfl_xor(sum, off, 64);
+          if (g64 == 0) finish(r, rs, re, sum);
+        }
+        seen++;
       }
     }
   }

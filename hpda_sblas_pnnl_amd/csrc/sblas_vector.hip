@@ -334,20 +334,56 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks_mrhs(
   }
   constexpr int SUBS = BLOCK / SUBW;
   constexpr int STEP = SUBW / K;
+  // a row of more than 16 steps of its lane groups goes to a second pass
+  // where a whole wave (64/K groups per vector) sums it: one long row no
+  // longer holds its sub-wave for len/STEP dependent steps
+  constexpr int LONG = 16 * STEP;
   const int sub = t / SUBW;
   const int grp = (t % SUBW) / K;
+  auto finish = [&](int r, int rs, int re, double sum) {
+    if (rs < rb.lo || re > rb.hi) {
+      partials[static_cast<int64_t>(2 * b + (r == rb.row_begin ? 0 : 1)) * K + vec] = sum;
+    } else {
+      double *yp = Y + static_cast<int64_t>(r) * K + vec;
+      *yp = (beta == 0.0) ? alpha * sum : alpha * sum + beta * (*yp);
+    }
+  };
   for (int r = rb.row_begin + sub; r < rb.row_end; r += SUBS) {
     const int rs = rp[r], re = rp[r + 1];
     const int s = max(rs, rb.lo), e = min(re, rb.hi);
+    if (SUBW < 64 && e - s > LONG) continue;
     double sum = strided_dot<VT, K>(s + grp, e, STEP, val, col, X, vec);
 #pragma unroll
     for (int off = SUBW / 2; off >= K; off >>= 1) sum += __shfl_xor(sum, off, SUBW);
-    if (grp == 0) {
-      if (rs < rb.lo || re > rb.hi) {
-        partials[static_cast<int64_t>(2 * b + (r == rb.row_begin ? 0 : 1)) * K + vec] = sum;
-      } else {
-        double *yp = Y + static_cast<int64_t>(r) * K + vec;
-        *yp = (beta == 0.0) ? alpha * sum : alpha * sum + beta * (*yp);
+    if (grp == 0) finish(r, rs, re, sum);
+  }
+  if constexpr (SUBW < 64) {
+    // long rows in block order, found by a ballot (the same list in every
+    // wave) and dealt out to the waves in turn
+    constexpr int WAVES = BLOCK / 64;
+    const int w = t >> 6, l = t & 63, g64 = l / K;
+    const int nrows = rb.row_end - rb.row_begin;
+    int seen = 0;
+    for (int base = 0; base < nrows; base += 64) {
+      bool is_long = false;
+      if (base + l < nrows) {
+        const int r = rb.row_begin + base + l;
+        is_long = min(rp[r + 1], rb.hi) - max(rp[r], rb.lo) > LONG;
+      }
+      uint64_t mask = __ballot(is_long);
+      while (mask) {
+        const int bit = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        mask &= mask - 1;
+        if (seen % WAVES == w) {
+          const int r = rb.row_begin + base + bit;
+          const int rs = rp[r], re = rp[r + 1];
+          double sum = strided_dot<VT, K>(max(rs, rb.lo) + g64, min(re, rb.hi), 64 / K, val, col,
+                                          X, vec);
+#pragma unroll
+          for (int off = 32; off >= K; off >>= 1) sum += __shfl_xor(sum, off, 64);
+          if (g64 == 0) finish(r, rs, re, sum);
+        }
+        seen++;
       }
     }
   }

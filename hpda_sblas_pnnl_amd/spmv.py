@@ -29,119 +29,113 @@ __all__ = [
 
 
 def _f64(a):
-    a = np.ascontiguousarray(a, dtype=np.float64)
-    return a
+    return np.ascontiguousarray(a, dtype=np.float64)
 
 
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
-def _y(y):
-    if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous):
-        raise TypeError("y must be a C-contiguous float64 numpy array (updated in place)")
-    return y
-
-
 def _scal(v):
     return ctypes.byref(ctypes.c_double(float(v)))
 
 
-def _call(name, *args):
-    ret = getattr(lib(), name)(*args)
+def _dropin(name, m, n, nnz, alpha, val, a0, a1, x, beta, y, *tail, mapping=None):
+    """Calls entry point `name` with the reference's argument order: sizes,
+    &alpha, the three matrix arrays, x, &beta, y, then the entry's own
+    trailing integers (ngpu, kernel, ...); a numa_mapping array, when given,
+    goes right after ngpu and kernel.  The converted arrays stay referenced
+    here until the call returns (ctypes sees only their addresses)."""
+    if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous):
+        raise TypeError("y must be a C-contiguous float64 numpy array (updated in place)")
+    val, a0, a1, x = _f64(val), _i32(a0), _i32(a1), _f64(x)
+    tail = list(tail)
+    if mapping is not None:
+        mp = _i32(mapping)
+        tail.insert(2, mp.ctypes.data)
+    ret = getattr(lib(), name)(m, n, nnz, _scal(alpha), val.ctypes.data, a0.ctypes.data,
+                               a1.ctypes.data, x.ctypes.data, _scal(beta), y.ctypes.data, *tail)
     if any(math.isnan(getattr(ret, f)) for f, _ in _lib.SpmvRet._fields_):
         raise SblasError(lib().sblas_last_error(), name)
     return ret
 
 
-def _csr_args(m, n, nnz, alpha, val, rp, col, x, beta, y):
-    val, rp, col, x, y = _f64(val), _i32(rp), _i32(col), _f64(x), _y(y)
-    keep = (val, rp, col, x, y)
-    return keep, (m, n, nnz, _scal(alpha), val.ctypes.data, rp.ctypes.data, col.ctypes.data,
-                  x.ctypes.data, _scal(beta), y.ctypes.data)
-
-
 def spMV_mgpu_baseline(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1):
     """Row-block CSR SpMV (src/dspmv_mgpu_baseline.cu:14-202)."""
-    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
-    return _call("spMV_mgpu_baseline", *a, ngpu)
+    return _dropin("spMV_mgpu_baseline", m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x,
+                   beta, y, ngpu)
 
 
 def spMV_mgpu_baseline_csc(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIdx, x, beta, y, ngpu=1):
     """Column-block CSC SpMV (src/dspmv_mgpu_baseline_csc.cu:14-239)."""
-    keep, a = _csr_args(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIdx, x, beta, y)
-    return _call("spMV_mgpu_baseline_csc", *a, ngpu)
+    return _dropin("spMV_mgpu_baseline_csc", m, n, nnz, alpha, cscVal, cscColPtr, cscRowIdx, x,
+                   beta, y, ngpu)
 
 
 def spMV_mgpu_baseline_coo(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y, ngpu=1):
     """Row-block COO SpMV (src/dspmv_mgpu_baseline_coo.cu:16-264)."""
-    keep, a = _csr_args(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y)
-    return _call("spMV_mgpu_baseline_coo", *a, ngpu)
+    return _dropin("spMV_mgpu_baseline_coo", m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x,
+                   beta, y, ngpu)
 
 
 def spMV_mgpu_v1(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
                  kernel=1):
     """nnz-chunked CSR SpMV with kernel switch 1/2/3 (src/dspmv_mgpu_v1.cu:76-360)."""
-    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
-    return _call("spMV_mgpu_v1", *a, ngpu, kernel)
-
-
-def _mapping(numa_mapping, ngpu):
-    mp = _i32(numa_mapping if numa_mapping is not None else [0] * max(ngpu, 1))
-    return mp
+    return _dropin("spMV_mgpu_v1", m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y,
+                   ngpu, kernel)
 
 
 def spMV_mgpu_v1_numa(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
                       kernel=1, numa_mapping=None, part_opt=0, merg_opt=0):
     """Two-level nnz-chunked CSR SpMV (src/dspmv_mgpu_v1_numa.cu:98-837)."""
-    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
-    mp = _mapping(numa_mapping, ngpu)
-    return _call("spMV_mgpu_v1_numa", *a, ngpu, kernel, mp.ctypes.data, part_opt, merg_opt)
+    return _dropin("spMV_mgpu_v1_numa", m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x,
+                   beta, y, ngpu, kernel, part_opt, merg_opt,
+                   mapping=numa_mapping if numa_mapping is not None else [0] * max(ngpu, 1))
 
 
 def spMV_mgpu_v1_numa_csc(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIndex, x, beta, y, ngpu=1,
                           kernel=1, numa_mapping=None, part_opt=0, merg_opt=0):
     """Two-level nnz-chunked CSC SpMV (src/dspmv_mgpu_v1_numa_csc.cu:43-483)."""
-    keep, a = _csr_args(m, n, nnz, alpha, cscVal, cscColPtr, cscRowIndex, x, beta, y)
-    mp = _mapping(numa_mapping, ngpu)
-    return _call("spMV_mgpu_v1_numa_csc", *a, ngpu, kernel, mp.ctypes.data, part_opt, merg_opt)
+    return _dropin("spMV_mgpu_v1_numa_csc", m, n, nnz, alpha, cscVal, cscColPtr, cscRowIndex, x,
+                   beta, y, ngpu, kernel, part_opt, merg_opt,
+                   mapping=numa_mapping if numa_mapping is not None else [0] * max(ngpu, 1))
 
 
 def spMV_mgpu_v1_numa_coo(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y, ngpu=1,
                           kernel=1, numa_mapping=None, part_opt=0, merg_opt=0):
     """Two-level nnz-chunked COO SpMV (src/dspmv_mgpu_v1_numa_coo.cu:35-633)."""
-    keep, a = _csr_args(m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x, beta, y)
-    mp = _mapping(numa_mapping, ngpu)
-    return _call("spMV_mgpu_v1_numa_coo", *a, ngpu, kernel, mp.ctypes.data, part_opt, merg_opt)
+    return _dropin("spMV_mgpu_v1_numa_coo", m, n, nnz, alpha, cooVal, cooRowIdx, cooColIdx, x,
+                   beta, y, ngpu, kernel, part_opt, merg_opt,
+                   mapping=numa_mapping if numa_mapping is not None else [0] * max(ngpu, 1))
 
 
 def spMV_mgpu_v2(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
                  kernel=1, nb=1 << 20, copy_of_workspace=1):
     """Task-queue CSR SpMV, nb nonzeros per task (src/dspmv_mgpu_v2.cu:103-300)."""
-    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
-    return _call("spMV_mgpu_v2", *a, ngpu, kernel, nb, copy_of_workspace)
+    return _dropin("spMV_mgpu_v2", m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y,
+                   ngpu, kernel, nb, copy_of_workspace)
 
 
 def spMspV_mgpu_v1(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
                    kernel=1):
     """v1 after dropping entries with x[col] == 0 (src/dspmv_mgpu_v1.cu:19-74)."""
-    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
-    return _call("spMspV_mgpu_v1", *a, ngpu, kernel)
+    return _dropin("spMspV_mgpu_v1", m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta,
+                   y, ngpu, kernel)
 
 
 def spMspV_mgpu_v1_numa(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
                         kernel=1):
     """Declared by the reference (include/spmv_kernel.h:282) but never defined:
     always raises."""
-    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
-    return _call("spMspV_mgpu_v1_numa", *a, ngpu, kernel)
+    return _dropin("spMspV_mgpu_v1_numa", m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x,
+                   beta, y, ngpu, kernel)
 
 
 def spMspV_mgpu_v2(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu=1,
                    kernel=1, nb=1 << 20, copy_of_workspace=1):
     """v2 after dropping entries with x[col] == 0 (src/dspmv_mgpu_v2.cu:35-101)."""
-    keep, a = _csr_args(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y)
-    return _call("spMspV_mgpu_v2", *a, ngpu, kernel, nb, copy_of_workspace)
+    return _dropin("spMspV_mgpu_v2", m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta,
+                   y, ngpu, kernel, nb, copy_of_workspace)
 
 
 def get_row_from_index(n, a, idx):
