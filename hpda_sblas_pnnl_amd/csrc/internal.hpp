@@ -94,9 +94,13 @@ int sorted_build(int val_type, int n, const int *col, const void *val,
 int launch_csr_sorted(int val_type, const SortedFormat &f, const int *rp, const double *x,
                       double *y, double alpha, double beta, hipStream_t s);
 
+// x32 (FP32 / BF16 values only): x gathered from this FP32 copy instead of x.
 int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *blocks,
                       const int *rp, const int *col, const void *val, const double *x,
-                      double *y, double alpha, double beta, double *partials, hipStream_t s);
+                      double *y, double alpha, double beta, double *partials, hipStream_t s,
+                      const float *x32 = nullptr);
+// x32[i] = x[i] rounded to FP32, i < n
+int launch_x_to_f32(int n, const double *x, float *x32, hipStream_t s);
 // k in {2, 4, 8, 16} right-hand sides over the same row blocks; partials hold
 // 2 slots x k per block.
 int launch_csr_blocks_mrhs(int val_type, int k, const StreamLaunch &L, const RowBlock *blocks,

@@ -105,6 +105,8 @@ struct Plan {
   // trial; the trial's median times (row blocks, CSR5-style) in ms
   int tuned = 0;
   float trial_ms[3] = {0.f, 0.f, 0.f};  // row blocks, CSR5-style, column-sorted row blocks
+  // FP32 copy of x for row blocks over narrow values (narrow_x = 2); nullptr: FP64 x
+  float *x32_rows = nullptr;
   int launches = 0;
 
   // kernel timing (measurement only)
@@ -127,6 +129,7 @@ struct Plan {
   int setup_tiles();
   int setup_csr5(const std::vector<int> &h_rp);
   int finish_csr(const std::vector<int> &h_rp);
+  int choose_csr(const std::vector<int> &h_rp);
   void free_buffers_from(size_t first);
   int setup_sorted(const int *h_rp);
   void drop_sorted();

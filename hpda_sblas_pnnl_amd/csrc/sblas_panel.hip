@@ -704,6 +704,13 @@ int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const dou
 
 }  // namespace
 
+int launch_x_to_f32(int n, const double *x, float *x32, hipStream_t s) {
+  if (n <= 0) return SBLAS_OK;
+  hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, x32);
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
+}
+
 int panel_rpt(int rows) {
   const int need = (rows + 255) / 256;
   return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : 8;

@@ -15,6 +15,8 @@
 // Matrix values and column indices are streamed with non-temporal loads so
 // they do not evict x from L2 / MALL.  Products and sums are FP64 for every
 // stored value type.
+#include <type_traits>
+
 #include "internal.hpp"
 #include "rows_device.hpp"
 
@@ -156,12 +158,14 @@ __global__ __launch_bounds__(BLOCK) void k_csr_vector_mrhs(
 // read from LDS) and the y writes (phase 3, consecutive threads on consecutive
 // rows) wait on no further memory round trip.  A block that holds a piece of
 // one long row sums in registers and reduces across the whole workgroup
-// instead.
-template <typename VT, int ITEMS, int SUBW, int BLOCK>
+// instead.  XT float: x gathered from its FP32 copy (narrow values: half the
+// bytes per gather and per cached line of x; an FP32 value times an FP32 x is
+// exact in FP64, so x's rounding is the only one added).
+template <typename VT, int ITEMS, int SUBW, int BLOCK, typename XT = double>
 __global__ __launch_bounds__(BLOCK) void k_csr_stream(
     const RowBlock *__restrict__ blocks, int nblocks, int xcd_remap,
     const int *__restrict__ rp, const int *__restrict__ col, const VT *__restrict__ val,
-    const double *__restrict__ x, double *__restrict__ y, double alpha, double beta,
+    const XT *__restrict__ x, double *__restrict__ y, double alpha, double beta,
     double *__restrict__ partials) {
   constexpr int CAP = BLOCK * ITEMS;
   constexpr int RPT = (kStreamRowsMax + BLOCK) / BLOCK;  // row pointers per thread
@@ -205,11 +209,12 @@ __global__ __launch_bounds__(BLOCK) void k_csr_stream(
       v[k] = load_val(val, e);
       c[k] = load_col(col, e);
     }
-    double xv[ITEMS];
+    XT xv[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) xv[k] = x[c[k]];
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) p[k] = (lo + t + k * BLOCK < hi) ? v[k] * xv[k] : 0.0;
+    for (int k = 0; k < ITEMS; k++)
+      p[k] = (lo + t + k * BLOCK < hi) ? v[k] * static_cast<double>(xv[k]) : 0.0;
   } else {
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) p[k] = 0.0;
@@ -499,11 +504,20 @@ int launch_mrhs_typed(int k, const VectorLaunch &L, const DevChunk *chunks, int 
 template <typename VT, int ITEMS>
 int launch_stream_items(const StreamLaunch &L, const RowBlock *blocks, const int *rp,
                         const int *col, const VT *val, const double *x, double *y,
-                        double alpha, double beta, double *partials, hipStream_t s) {
+                        double alpha, double beta, double *partials, hipStream_t s,
+                        const float *x32) {
   constexpr int B = kStreamBlock;
   const dim3 grid(L.grid), block(B);
 #define SBLAS_STREAM_CASE(W)                                                              \
   case W:                                                                                 \
+    if constexpr (!std::is_same<VT, double>::value) {                                     \
+      if (x32) {                                                                          \
+        hipLaunchKernelGGL((k_csr_stream<VT, ITEMS, W, B, float>), grid, block, 0, s,     \
+                           blocks, L.nblocks, L.xcd_remap, rp, col, val, x32, y, alpha,   \
+                           beta, partials);                                               \
+        break;                                                                            \
+      }                                                                                   \
+    }                                                                                     \
     hipLaunchKernelGGL((k_csr_stream<VT, ITEMS, W, B>), grid, block, 0, s, blocks,        \
                        L.nblocks, L.xcd_remap, rp, col, val, x, y, alpha, beta, partials); \
     break;
@@ -527,12 +541,15 @@ int launch_stream_items(const StreamLaunch &L, const RowBlock *blocks, const int
 template <typename VT>
 int launch_stream_typed(const StreamLaunch &L, const RowBlock *blocks, const int *rp,
                         const int *col, const VT *val, const double *x, double *y,
-                        double alpha, double beta, double *partials, hipStream_t s) {
+                        double alpha, double beta, double *partials, hipStream_t s,
+                        const float *x32) {
   switch (L.items) {
     case 8:
-      return launch_stream_items<VT, 8>(L, blocks, rp, col, val, x, y, alpha, beta, partials, s);
+      return launch_stream_items<VT, 8>(L, blocks, rp, col, val, x, y, alpha, beta, partials, s,
+                                        x32);
     case 16:
-      return launch_stream_items<VT, 16>(L, blocks, rp, col, val, x, y, alpha, beta, partials, s);
+      return launch_stream_items<VT, 16>(L, blocks, rp, col, val, x, y, alpha, beta, partials,
+                                         s, x32);
     default:
       set_error(SBLAS_ERR_ARGS, "stream kernel: items per thread must be 8 or 16");
       return SBLAS_ERR_ARGS;
@@ -610,18 +627,19 @@ int launch_csr_blocks_mrhs(int val_type, int k, const StreamLaunch &L, const Row
 
 int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *blocks,
                       const int *rp, const int *col, const void *val, const double *x,
-                      double *y, double alpha, double beta, double *partials, hipStream_t s) {
+                      double *y, double alpha, double beta, double *partials, hipStream_t s,
+                      const float *x32) {
   if (L.grid <= 0) return SBLAS_OK;
   switch (val_type) {
     case SBLAS_VAL_F64:
       return launch_stream_typed(L, blocks, rp, col, static_cast<const double *>(val), x, y,
-                                 alpha, beta, partials, s);
+                                 alpha, beta, partials, s, nullptr);
     case SBLAS_VAL_F32:
       return launch_stream_typed(L, blocks, rp, col, static_cast<const float *>(val), x, y,
-                                 alpha, beta, partials, s);
+                                 alpha, beta, partials, s, x32);
     case SBLAS_VAL_BF16:
       return launch_stream_typed(L, blocks, rp, col, static_cast<const uint16_t *>(val), x, y,
-                                 alpha, beta, partials, s);
+                                 alpha, beta, partials, s, x32);
     default:
       set_error(SBLAS_ERR_ARGS, "unknown value type");
       return SBLAS_ERR_ARGS;

@@ -213,12 +213,13 @@ typedef struct sblas_opts {
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
   int panel_rows;  /* column-panel form: rows per row panel (1..2048, or   */
                    /* 1..1536 in a batch; 0 auto)                           */
-  int narrow_x;    /* FP32 / BF16 values in the column-panel form: 0 auto   */
-                   /* (= on), 1 off: x staged in FP64, each row summed in   */
-                   /* the host loop's order; 2 on: x staged from an FP32    */
-                   /* copy, FP32 products summed in FP64 (error within a    */
-                   /* few FP32 roundings per product); in a batch such a    */
-                   /* matrix runs as a launch of its own                    */
+  int narrow_x;    /* FP32 / BF16 values, x in FP32: 0 auto (on for column  */
+                   /* panels, and for row blocks of >= 2^20 elements), 1    */
+                   /* off: x in FP64, each row summed in the host loop's    */
+                   /* order; 2 on: x staged or gathered from an FP32 copy   */
+                   /* written by each product (error within a few FP32      */
+                   /* roundings per product); in a batch a narrow-x matrix  */
+                   /* in column-panel form runs as a launch of its own      */
   int panel_layout;  /* column-panel form of a plan, layout inside a slice: */
                      /* 0 auto (2 where x is staged in FP32, else 1), 1    */
                      /* rows in order (one thread per row sums staged      */

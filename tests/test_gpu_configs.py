@@ -143,25 +143,44 @@ def test_rmat22_dropins(torch_dev, oracle, rmat, rmat_ref, entry, monkeypatch):
     assert oracle.err_normwise(y, rmat_ref) <= TOL, entry
 
 
+def _narrow_check(oracle, plan, A, vr, x, y0, got, ref):
+    """x gathered or staged from an FP32 copy adds x's rounding: every row
+    within 2^-22 of sum |a_ij x_j|; else within 1e-12 of the rounded-value
+    oracle."""
+    if plan.info()["narrow_x"]:
+        assert oracle.err_maxrow(A.m, vr, A.row_ptr, A.col, x, 1.0, 0.5, y0, got, ref) <= 2.0**-22
+        assert oracle.err_normwise(got, ref) <= TOL_FP32_VS_FP64
+    else:
+        assert oracle.err_normwise(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("nx", [0, 1])
 @pytest.mark.parametrize("vt", ["f32", "bf16"])
-def test_rmat22_narrow_values(torch_dev, oracle, rmat, rmat_ref, vt):
-    """Binary values are exact in FP32 and BF16: the narrow forms meet the
-    FP64 oracle at the FP64 bar, and the rounded-value oracles agree."""
+def test_rmat22_narrow_values(torch_dev, oracle, rmat, rmat_ref, vt, nx):
+    """Binary values are exact in FP32 and BF16, so the rounded-value oracles
+    equal the FP64 one; with x in FP64 (narrow_x 1) the narrow forms meet it
+    at the FP64 bar, with x from an FP32 copy (the default) within x's
+    rounding."""
     from hpda_sblas_pnnl_amd import gen as G
     from hpda_sblas_pnnl_amd.device import SpmvPlan
     torch = torch_dev
     A, x, y0 = rmat
     if vt == "f32":
         vals = A.val.astype(np.float32)
+        vr = vals.astype(np.float64)
         ref = oracle.csr_spmv_f32val(A.m, vals, A.row_ptr, A.col, x, 1.0, 0.5, y0)
     else:
         bits = G.to_bf16_bits(A.val)
         vals = bits.view(np.int16)
+        vr = G.from_bf16_bits(bits)
         ref = oracle.csr_spmv_bf16val(A.m, bits, A.row_ptr, A.col, x, 1.0, 0.5, y0)
     assert np.array_equal(ref, rmat_ref)
     rp, col, dv = _dev(torch, A.row_ptr, A.col, vals)
-    got = _plan_run(torch, SpmvPlan("csr", A.m, A.n, rp, col, dv), x, y0, 1.0, 0.5)
-    assert oracle.err_normwise(got, ref) <= TOL
+    plan = SpmvPlan("csr", A.m, A.n, rp, col, dv, narrow_x=nx)
+    got = _plan_run(torch, plan, x, y0, 1.0, 0.5)
+    if nx == 1:
+        assert plan.info()["narrow_x"] == 0
+    _narrow_check(oracle, plan, A, vr, x, y0, got, ref)
 
 
 def test_rmat22_spmm_k8(torch_dev, oracle, rmat):
@@ -200,14 +219,17 @@ def test_powerlaw_narrow_values(torch_dev, oracle, vt):
     ref64 = oracle.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, 1.0, 0.5, y0)
     if vt == "f32":
         vals = A.val.astype(np.float32)
+        vr = vals.astype(np.float64)
         ref = oracle.csr_spmv_f32val(A.m, vals, A.row_ptr, A.col, x, 1.0, 0.5, y0)
     else:
         bits = G.to_bf16_bits(A.val)
         vals = bits.view(np.int16)
+        vr = G.from_bf16_bits(bits)
         ref = oracle.csr_spmv_bf16val(A.m, bits, A.row_ptr, A.col, x, 1.0, 0.5, y0)
     rp, col, dv = _dev(torch, A.row_ptr, A.col, vals)
-    got = _plan_run(torch, SpmvPlan("csr", A.m, A.n, rp, col, dv), x, y0, 1.0, 0.5)
-    assert oracle.err_normwise(got, ref) <= TOL
+    plan = SpmvPlan("csr", A.m, A.n, rp, col, dv)
+    got = _plan_run(torch, plan, x, y0, 1.0, 0.5)
+    _narrow_check(oracle, plan, A, vr, x, y0, got, ref)
     if vt == "f32":
         assert oracle.err_normwise(got, ref64) <= TOL_FP32_VS_FP64
 
