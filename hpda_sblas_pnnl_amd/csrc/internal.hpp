@@ -259,7 +259,19 @@ struct BatchMatrix {
   const uint16_t *pcol;
   int prows;    // rows per row panel
   int k;        // right-hand sides (x, y are n x k, m x k row-major); 1 = one vector
+  // row blocks over FP32 / BF16 values: x gathered from this FP32 copy,
+  // written at the start of each run (null: from x)
+  const float *x32;
 };
+// One x converted to its FP32 copy at the start of a batch run: entries
+// [start, start + n) of the launch's index space.
+struct BatchX32 {
+  const double *x;
+  float *x32;
+  int64_t start;
+};
+// x32[i] = x[i] rounded to FP32 for every segment (total entries in all).
+int launch_batch_x32(int nseg, const BatchX32 *segs, int64_t total, hipStream_t s);
 constexpr int kBatchRowBlock = 0;
 constexpr int kBatchPanel = 1;
 struct BatchItem {  // one row block, or one row panel, of one matrix
@@ -275,11 +287,12 @@ struct BatchItem {  // one row block, or one row panel, of one matrix
 // [qbound[q], qbound[q+1]); workgroups take items from per-queue counters
 // ctr[q] (ctr[kBatchQueues] counts finished workgroups; the last one resets
 // them all).  Each item's arithmetic does not depend on which workgroup runs it.
+// flat: one item per workgroup instead (grid, qbound and ctr unused).
 constexpr int kBatchQueues = 8;  // one per XCD
 constexpr int kBatchPanelRowsMax = 1536;  // rows per row panel in a batch (6 per thread)
 int launch_batch(int kind, int items_per_thread, int grid, int nitems, const BatchItem *items,
                  const BatchMatrix *mats, double *partials, const int *qbound, int *ctr,
-                 hipStream_t s);
+                 hipStream_t s, bool flat = false);
 // Resident workgroups per CU of the batched kernel for one item kind.
 int batch_occupancy(int kind, int items_per_thread);
 int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,

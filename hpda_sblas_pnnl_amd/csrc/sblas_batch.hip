@@ -61,8 +61,18 @@ __device__ __forceinline__ void row_block(int w, const BatchItem &it, const Batc
       c[k] = __builtin_nontemporal_load(col + e);
     }
     double xv[ITEMS];
+    if (sizeof(VT) < 8 && M.x32) {
+      // FP32 copy: 4-byte gathers, half the lines of x; an FP32 value times
+      // an FP32 x is exact in FP64, so x's rounding is the only one added
+      float xf[ITEMS];
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) xv[k] = x[c[k]];
+      for (int k = 0; k < ITEMS; k++) xf[k] = M.x32[c[k]];
+#pragma unroll
+      for (int k = 0; k < ITEMS; k++) xv[k] = static_cast<double>(xf[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < ITEMS; k++) xv[k] = x[c[k]];
+    }
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) p[k] = (lo + t + k * BLOCK < hi) ? v[k] * xv[k] : 0.0;
   } else {
@@ -306,6 +316,32 @@ __device__ __forceinline__ void rows_typed(int w, const BatchItem &it, const Bat
 // row blocks).  Each kind is its own kernel compiled with only its own body,
 // so its registers and LDS (52 KB for panels, 20-36 KB for row blocks) set
 // its own occupancy.
+// LDS of one workgroup of a batch launch, in doubles
+template <int MODE, int ITEMS, int BLOCK>
+__host__ __device__ constexpr int batch_lds() {
+  return MODE == kBatchPanel ? panel::lds_doubles<uint16_t, kBatchPanelRowsMax / 256>()
+                             : BLOCK * ITEMS + kStreamRowsMax;
+}
+
+// work item w of the list
+template <int MODE, int ITEMS, int BLOCK>
+__device__ __forceinline__ void batch_item(int w, const BatchItem *__restrict__ items,
+                                           const BatchMatrix *__restrict__ mats,
+                                           double *__restrict__ partials, double *lds,
+                                           double *wsum) {
+  const BatchItem it = items[w];
+  const BatchMatrix &M = mats[it.mat];
+  if constexpr (MODE == kBatchPanel) {
+    row_panel_any(it, M, partials, lds);
+  } else if (M.val_type == SBLAS_VAL_F64) {  // value type chosen once per item
+    rows_typed<double, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+  } else if (M.val_type == SBLAS_VAL_F32) {
+    rows_typed<float, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+  } else {
+    rows_typed<uint16_t, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
+  }
+}
+
 template <int MODE, int ITEMS, int BLOCK>
 __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
                                            const BatchMatrix *__restrict__ mats,
@@ -313,10 +349,7 @@ __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
                                            const int *__restrict__ qbound,
                                            int *__restrict__ ctr) {
   static_assert(BLOCK == 256, "the panel body assumes 256 threads");
-  constexpr int LDS = MODE == kBatchPanel
-                          ? panel::lds_doubles<uint16_t, kBatchPanelRowsMax / 256>()
-                          : BLOCK * ITEMS + kStreamRowsMax;
-  __shared__ double lds[LDS];
+  __shared__ double lds[batch_lds<MODE, ITEMS, BLOCK>()];
   __shared__ double wsum[BLOCK / 64];
   __shared__ int next;
   const int t = threadIdx.x;
@@ -336,17 +369,7 @@ __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
       // claim the following item now; its counter round trip overlaps this item
       int claimed = 0;
       if (t == 0) claimed = qbound[q] + atomicAdd(&ctr[q], 1);
-      const BatchItem it = items[w];
-      const BatchMatrix &M = mats[it.mat];
-      if constexpr (MODE == kBatchPanel) {
-        row_panel_any(it, M, partials, lds);
-      } else if (M.val_type == SBLAS_VAL_F64) {  // value type chosen once per item
-        rows_typed<double, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
-      } else if (M.val_type == SBLAS_VAL_F32) {
-        rows_typed<float, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
-      } else {
-        rows_typed<uint16_t, ITEMS, BLOCK>(w, it, M, partials, lds, wsum);
-      }
+      batch_item<MODE, ITEMS, BLOCK>(w, items, mats, partials, lds, wsum);
       if (t == 0) next = claimed;  // last read before this iteration's first barrier
       __syncthreads();
       w = next;
@@ -371,6 +394,39 @@ __global__ __launch_bounds__(BLOCK) void k_batch(int nitems, const BatchItem *__
   (void)nitems;
 }
 
+// Entry i of the segments' joint index space: the segment by a binary search
+// over their starts (ascending, segs[0].start == 0), then x32 = FP32 of x.
+__global__ void k_batch_x32(int nseg, const BatchX32 *__restrict__ segs, int64_t total) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].start <= i) lo = mid; else hi = mid - 1;
+  }
+  const int64_t j = i - segs[lo].start;
+  segs[lo].x32[j] = static_cast<float>(segs[lo].x[j]);
+}
+
+// One work item per workgroup, for a launch that shares the GPU with others:
+// the hardware hands out its workgroups as CUs free up, where a persistent
+// launch would hold every CU until its queues drain.  Workgroup b runs on
+// XCD b % 8 and takes item (b % 8) * G/8 + b / 8, so each XCD works through a
+// contiguous run of the list (the grid is a multiple of 8).
+template <int MODE, int ITEMS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_flat(int nitems,
+                                                      const BatchItem *__restrict__ items,
+                                                      const BatchMatrix *__restrict__ mats,
+                                                      double *__restrict__ partials) {
+  static_assert(BLOCK == 256, "the panel body assumes 256 threads");
+  __shared__ double lds[batch_lds<MODE, ITEMS, BLOCK>()];
+  __shared__ double wsum[BLOCK / 64];
+  const int per = static_cast<int>(gridDim.x) >> 3;
+  const int w = (static_cast<int>(blockIdx.x) & 7) * per + (static_cast<int>(blockIdx.x) >> 3);
+  if (w >= nitems) return;
+  batch_item<MODE, ITEMS, BLOCK>(w, items, mats, partials, lds, wsum);
+}
+
 // Split rows of all matrices: one thread per row (all its vectors), partials
 // in item order.
 __global__ void k_batch_merge(int groups, const int *__restrict__ gmat,
@@ -393,10 +449,22 @@ __global__ void k_batch_merge(int groups, const int *__restrict__ gmat,
 
 int launch_batch(int kind, int items_per_thread, int grid, int nitems, const BatchItem *items,
                  const BatchMatrix *mats, double *partials, const int *qbound, int *ctr,
-                 hipStream_t s) {
+                 hipStream_t s, bool flat) {
   if (nitems <= 0 || grid <= 0) return SBLAS_OK;
   constexpr int B = kStreamBlock;
   constexpr int P = kBatchPanel, R = kBatchRowBlock;
+  if (flat) {
+    const dim3 g(static_cast<unsigned>((nitems + 7) / 8 * 8));
+    if (kind == kBatchPanel)
+      hipLaunchKernelGGL((k_batch_flat<P, 8, B>), g, dim3(B), 0, s, nitems, items, mats, partials);
+    else if (items_per_thread == 16)
+      hipLaunchKernelGGL((k_batch_flat<R, 16, B>), g, dim3(B), 0, s, nitems, items, mats,
+                         partials);
+    else
+      hipLaunchKernelGGL((k_batch_flat<R, 8, B>), g, dim3(B), 0, s, nitems, items, mats, partials);
+    SBLAS_HIP_TRY(hipGetLastError());
+    return SBLAS_OK;
+  }
   if (kind == kBatchPanel)
     hipLaunchKernelGGL((k_batch<P, 8, B>), dim3(grid), dim3(B), 0, s, nitems, items, mats,
                        partials, qbound, ctr);
@@ -421,6 +489,14 @@ int batch_occupancy(int kind, int items_per_thread) {
       per_cu < 1)
     per_cu = 1;
   return per_cu;
+}
+
+int launch_batch_x32(int nseg, const BatchX32 *segs, int64_t total, hipStream_t s) {
+  if (nseg <= 0 || total <= 0) return SBLAS_OK;
+  hipLaunchKernelGGL(k_batch_x32, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+                     s, nseg, segs, total);
+  SBLAS_HIP_TRY(hipGetLastError());
+  return SBLAS_OK;
 }
 
 int launch_batch_merge(int groups, const int *gmat, const int *grow, const int *gptr,

@@ -429,8 +429,17 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
   // row-parallel block of 2048 elements).  Measured on banded matrices: row
   // panels spanning 3.7, 5.2 and 7.7 column panels run 1.2x, 1.4x and 2.1x
   // faster than row blocks, 2.1 column panels 0.7x.
+  // One workgroup sums a row panel, so a panel holding far more than its
+  // share keeps its workgroup running long after the others (the first row
+  // panels of an R-MAT matrix hold most of its elements; in the 64-matrix
+  // batch ten such matrices held the persistent launch for 1.96 ms): the
+  // densest panel may hold at most 4x the mean, or 65536 elements.
+  int64_t densest = 0;
+  for (const int2 &w : hwin) densest = std::max<int64_t>(densest, w.y - w.x);
+  const bool balanced =
+      densest <= std::max<int64_t>(65536, 4 * nnz / std::max(f.npanels, 1));
   const bool helps = f.npanels >= min_panels && f.avg_slice >= 1024.0 &&
-                     f.avg_panels_per_row_panel > 3.0;
+                     f.avg_panels_per_row_panel > 3.0 && balanced;
   if (!force && !helps) {
     for (void *p : f.buffers) (void)hipFree(p);
     f.buffers.clear();

@@ -329,8 +329,9 @@ def test_batch64_fp32_graph(torch_dev, oracle):
         torch.cuda.synchronize()
         outs.append([it["y"].cpu().numpy() for it in items])
     for i, (kind, A) in enumerate(mats):
-        if batch.choice(i)["variant"] == 3:
-            # column panels stage x in FP32: each row within two FP32
+        if batch.choice(i)["variant"] == 3 or A.nnz >= 2**20:
+            # column panels stage x in FP32, row blocks of 2^20 elements or
+            # more gather it from an FP32 copy: each row within two FP32
             # roundings per product of the FP32-value oracle
             vr = A.val.astype(np.float32).astype(np.float64)
             x = G.values(A.n, i, 2, "uniform")
