@@ -75,6 +75,11 @@ struct sblas_batch_s {
   std::vector<sblas::BatchX32> xsegs;
   sblas::BatchX32 *d_xsegs = nullptr;
   int64_t x32_total = 0;
+  // and those of the matrices with launches of their own (one launch ahead
+  // of theirs)
+  std::vector<sblas::BatchX32> solo_xsegs;
+  sblas::BatchX32 *d_solo_xsegs = nullptr;
+  int64_t solo_x32_total = 0;
   // item list: [0, n_panel) row panels in row order, then row blocks
   int n_panel = 0;
   int grid_panel = 0, grid_rows = 0;
@@ -150,10 +155,14 @@ struct sblas_batch_s {
                              d_items + n_panel, d_mats, d_partials, d_qbound + Q, d_ctr + Q, rs,
                              flat >= 1);
     if (st != SBLAS_OK) return st;
+    // their FP32 copies of x in one launch, then one launch each
+    st = sblas::launch_batch_x32(static_cast<int>(solo_xsegs.size()), d_solo_xsegs,
+                                 solo_x32_total, s);
+    if (st != SBLAS_OK) return st;
     for (int i : solo) {
       const sblas::BatchMatrix &M = mats[i];
       st = sblas::launch_panel_spmv(M.val_type, panels[i], M.m, M.n, 1, M.x, M.y, M.alpha, M.beta,
-                                    s);
+                                    s, true);
       if (st != SBLAS_OK) return st;
     }
     if (used > 1) {
@@ -359,6 +368,8 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
           return fail(SBLAS_ERR_HIP);
         }
         f.x32 = static_cast<float *>(p);
+        b->solo_xsegs.push_back(BatchX32{it.d_x, f.x32, b->solo_x32_total});
+        b->solo_x32_total += it.n;
       }
     }
     b->mats.push_back(M);
@@ -490,7 +501,8 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       (st = b->upload(&b->d_grow, grow)) != SBLAS_OK ||
       (st = b->upload(&b->d_gptr, gptr)) != SBLAS_OK ||
       (st = b->upload(&b->d_gslot, gslot)) != SBLAS_OK ||
-      (st = b->upload(&b->d_xsegs, b->xsegs)) != SBLAS_OK)
+      (st = b->upload(&b->d_xsegs, b->xsegs)) != SBLAS_OK ||
+      (st = b->upload(&b->d_solo_xsegs, b->solo_xsegs)) != SBLAS_OK)
     return fail(st);
   std::vector<double> zeros(static_cast<size_t>(std::max<int64_t>(nslots, 1)), 0.0);
   if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);
@@ -564,8 +576,8 @@ int sblas_batch_get_shape(sblas_batch b, int *work_items, int *grid, int *launch
   if (grid) *grid = std::max(b->grid_panel, b->grid_rows);
   if (launches)
     *launches = (b->grid_panel > 0 ? 1 : 0) + (b->grid_rows > 0 ? 1 : 0) +
-                (b->groups > 0 ? 1 : 0) + 2 * static_cast<int>(b->solo.size()) +
-                (b->xsegs.empty() ? 0 : 1);
+                (b->groups > 0 ? 1 : 0) + static_cast<int>(b->solo.size()) +
+                (b->solo.empty() ? 0 : 1) + (b->xsegs.empty() ? 0 : 1);
   return SBLAS_OK;
 }
 

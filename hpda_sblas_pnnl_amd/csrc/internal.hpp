@@ -236,8 +236,10 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
                 const void *val, const std::vector<ExecChunk> &ex, const int *h_rp,
                 bool force, int min_panels, PanelFormat &f, bool *chosen, hipStream_t s,
                 int layout = kPanelRowOrder);
+// x32_ready: the FP32 copy of x (narrow-x form) is already written.
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
-                      const double *x, double *y, double alpha, double beta, hipStream_t s);
+                      const double *x, double *y, double alpha, double beta, hipStream_t s,
+                      bool x32_ready = false);
 
 // batched matrices (sblas_batch.hip)
 struct BatchMatrix {

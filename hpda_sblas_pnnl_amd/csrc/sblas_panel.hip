@@ -695,12 +695,14 @@ int sorted_launch(const PanelFormat &f, int n, int xcd_remap, const double *x, d
 
 template <typename VT>
 int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const double *x, double *y,
-                    double alpha, double beta, hipStream_t s) {
+                    double alpha, double beta, hipStream_t s, bool x32_ready) {
   if constexpr (!std::is_same<VT, double>::value) {
     if (f.x32) {
       // the FP32 copy of x first, on the same stream
-      hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, f.x32);
-      SBLAS_HIP_TRY(hipGetLastError());
+      if (!x32_ready) {
+        hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, f.x32);
+        SBLAS_HIP_TRY(hipGetLastError());
+      }
       if (f.layout == kPanelSorted)
         return sorted_launch<VT, true>(f, n, xcd_remap, x, y, alpha, beta, s);
       return panel_launch_nx<VT, true>(f, m, n, xcd_remap, x, y, alpha, beta, s);
@@ -809,12 +811,16 @@ int panel_rows_for(int val_type, int m, int device, bool narrow_x, int layout, i
 }
 
 int launch_panel_spmv(int val_type, const PanelFormat &f, int m, int n, int xcd_remap,
-                      const double *x, double *y, double alpha, double beta, hipStream_t s) {
+                      const double *x, double *y, double alpha, double beta, hipStream_t s,
+                      bool x32_ready) {
   if (f.npanels <= 0) return SBLAS_OK;
   switch (val_type) {
-    case SBLAS_VAL_F64: return panel_launch_vt<double>(f, m, n, xcd_remap, x, y, alpha, beta, s);
-    case SBLAS_VAL_F32: return panel_launch_vt<float>(f, m, n, xcd_remap, x, y, alpha, beta, s);
-    default: return panel_launch_vt<uint16_t>(f, m, n, xcd_remap, x, y, alpha, beta, s);
+    case SBLAS_VAL_F64:
+      return panel_launch_vt<double>(f, m, n, xcd_remap, x, y, alpha, beta, s, x32_ready);
+    case SBLAS_VAL_F32:
+      return panel_launch_vt<float>(f, m, n, xcd_remap, x, y, alpha, beta, s, x32_ready);
+    default:
+      return panel_launch_vt<uint16_t>(f, m, n, xcd_remap, x, y, alpha, beta, s, x32_ready);
   }
 }
 
