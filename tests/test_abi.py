@@ -110,3 +110,38 @@ def test_exit_on_error_emulation():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
                        timeout=120)
     assert r.returncode == 1 and "sblas error 2" in r.stderr, r.stdout + r.stderr
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every struct of include/sblas.h, compiled as C, has the size and field
+    offsets of its ctypes mirror in the Python package (the Python side reads
+    and writes these structs through pointers)."""
+    import shutil
+    import subprocess
+    from hpda_sblas_pnnl_amd import _lib
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    mirrors = {"spmv_ret": _lib.SpmvRet, "sblas_chunk_desc": _lib.ChunkDesc,
+               "sblas_opts": _lib.Opts, "sblas_plan_info": _lib.PlanInfo,
+               "sblas_batch_item": _lib.BatchItem, "sblas_batch_choice": _lib.BatchChoice}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "sblas.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} . %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    seen = 0
+    for line in filter(None, out):
+        cname, fname, value = line.split()
+        py = mirrors[cname]
+        expect = ctypes.sizeof(py) if fname == "." else getattr(py, fname).offset
+        assert int(value) == expect, (cname, fname, int(value), expect)
+        seen += 1
+    assert seen == sum(len(py._fields_) + 1 for py in mirrors.values())
