@@ -312,10 +312,6 @@ __device__ __forceinline__ void rows_typed(int w, const BatchItem &it, const Bat
   }
 }
 
-// The persistent loop of one item kind (MODE kBatchPanel: row panels, else
-// row blocks).  Each kind is its own kernel compiled with only its own body,
-// so its registers and LDS (52 KB for panels, 20-36 KB for row blocks) set
-// its own occupancy.
 // LDS of one workgroup of a batch launch, in doubles
 template <int MODE, int ITEMS, int BLOCK>
 __host__ __device__ constexpr int batch_lds() {
@@ -342,6 +338,10 @@ __device__ __forceinline__ void batch_item(int w, const BatchItem *__restrict__ 
   }
 }
 
+// The persistent loop of one item kind (MODE kBatchPanel: row panels, else
+// row blocks).  Each kind is its own kernel compiled with only its own body,
+// so its registers and LDS (52 KB for panels, 20-36 KB for row blocks) set
+// its own occupancy.
 template <int MODE, int ITEMS, int BLOCK>
 __device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items,
                                            const BatchMatrix *__restrict__ mats,
