@@ -307,14 +307,18 @@ int sblas_plan_set_timing(sblas_plan plan, int on);
 int sblas_plan_kernel_times(sblas_plan plan, float *ms, int cap);
 
 /* ---------------------------------------------------------------------- */
-/* batched matrices: persistent launches, replayable as one hipGraph       */
+/* batched matrices: launches over work lists, replayable as one hipGraph  */
 /* ---------------------------------------------------------------------- */
 /* No counterpart in the reference (its calls take one matrix).  Each item is
  * a CSR matrix on the device with its own x, y, alpha, beta; arrays must stay
- * alive and unchanged while the batch exists.  A run is one persistent launch
- * over the matrices' row panels and one over their row blocks (side by side,
- * the second on a stream forked from the caller's and joined back), then one
- * merge launch.  A batch runs one call at a time. */
+ * alive and unchanged while the batch exists.  A run is a launch of its own
+ * for each large FP32 / BF16 matrix in column-panel form (on the caller's
+ * stream, one after another), a persistent launch over the other matrices'
+ * row panels and one over their row blocks (one work item per workgroup when
+ * launches of their own run beside it; each on a stream forked from the
+ * caller's and joined back; narrow row-block matrices of 2^20 elements or
+ * more gather x from FP32 copies written by one launch ahead of them), then
+ * one merge launch.  A batch runs one call at a time. */
 typedef struct sblas_batch_item {
   int m, n;
   int64_t nnz;
@@ -353,8 +357,8 @@ int sblas_batch_run(sblas_batch batch, void *stream);
 /* The same work as one hipGraph (captured on first use, replayed after). */
 int sblas_batch_graph_launch(sblas_batch batch, void *stream);
 int sblas_batch_get_choice(sblas_batch batch, int index, sblas_batch_choice *out);
-/* work items, the largest persistent grid of the launches (sorted-layout row
- * panels, row-order row panels, row blocks), kernel launches per run */
+/* work items, the largest persistent grid of the work-list launches
+ * (row-order row panels, row blocks), kernel launches per run */
 int sblas_batch_get_shape(sblas_batch batch, int *work_items, int *grid, int *launches);
 int sblas_batch_destroy(sblas_batch batch);
 
