@@ -75,11 +75,13 @@ class PlanInfo(ctypes.Structure):
                 ("variant", _I), ("row_blocks", _I), ("items", _I),
                 ("panel_rows", _I), ("panel_slices", _I), ("tuned", _I),
                 ("trial_ms", ctypes.c_float * 3), ("panel_layout", _I), ("narrow_x", _I),
-                ("panel_elements", _I64)]
+                ("panel_elements", _I64), ("segments", _I), ("virtual_rows", _I),
+                ("trial_seg_ms", ctypes.c_float), ("reserved_info", _I)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "trial_ms"}
         d["trial_ms"] = [round(float(v), 4) for v in self.trial_ms]
+        d["trial_seg_ms"] = round(float(self.trial_seg_ms), 4)
         return d
 
 

@@ -30,6 +30,7 @@ HIP_SOURCES = [
     "sblas_batch.hip",
     "sblas_panel.hip",
     "sblas_sorted.hip",
+    "sblas_segment.hip",
     "sblas_convert.hip",
     "batch.cpp",
     "ingest.cpp",

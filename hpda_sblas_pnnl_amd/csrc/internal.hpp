@@ -101,6 +101,30 @@ int launch_csr_stream(int val_type, const StreamLaunch &L, const RowBlock *block
                       const float *x32 = nullptr);
 // x32[i] = x[i] rounded to FP32, i < n
 int launch_x_to_f32(int n, const double *x, float *x32, hipStream_t s);
+// Row blocks over column segments (sblas_segment.hip): the elements of row r
+// in column segment s (columns [ceil(s n/P), ceil((s+1) n/P))) form virtual
+// row (s, r); virtual rows holding elements are numbered segment-major, rows
+// ascending, and the matrix is copied in that order (vrp, vcol, vval).
+constexpr int kSegmentsMax = 64;
+struct SegmentFormat {
+  int P = 0;                    // column segments
+  int m = 0;                    // real rows
+  int M = 0;                    // virtual rows
+  int *vrp = nullptr;           // [M + 1] virtual row pointers
+  int *vcol = nullptr;          // [nnz] columns in virtual-row order
+  void *vval = nullptr;         // [nnz] values in virtual-row order
+  int *vptr = nullptr;          // [m + 1] real row r's virtual rows: vidx[vptr[r]..vptr[r+1])
+  int *vidx = nullptr;          // [M] their indices, segments ascending
+  double *yv = nullptr;         // [M] one sum per virtual row
+  std::vector<void *> buffers;
+  int64_t bytes = 0;
+};
+// Builds the format on the device (synchronous); h_vrp receives the virtual
+// row pointers.
+int seg_build(int val_type, int m, int n, int64_t nnz, const int *rp, const int *col,
+              const void *val, int P, SegmentFormat &f, std::vector<int> &h_vrp);
+// y[r] = alpha * (f.yv over r's virtual rows, segments ascending) + beta*y[r]
+int launch_seg_merge(const SegmentFormat &f, double alpha, double beta, double *y, hipStream_t s);
 // k in {2, 4, 8, 16} right-hand sides over the same row blocks; partials hold
 // 2 slots x k per block.
 int launch_csr_blocks_mrhs(int val_type, int k, const StreamLaunch &L, const RowBlock *blocks,

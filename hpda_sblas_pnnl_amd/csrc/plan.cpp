@@ -103,7 +103,7 @@ struct TuneKey {
 };
 struct TuneResult {
   int kernel, variant;
-  float ms[3];  // trial medians: row blocks, CSR5-style, column-sorted row blocks
+  float ms[4];  // trial medians: row blocks, CSR5-style, column-sorted row blocks, segments
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneResult> g_tune_cache;
@@ -122,6 +122,7 @@ TuneKey tune_key(int val_type, const RowStats &s, int64_t nnz) {
 Plan::~Plan() {
   for (void *p : panel.buffers) (void)hipFree(p);
   for (void *p : sorted.buffers) (void)hipFree(p);
+  for (void *p : seg.buffers) (void)hipFree(p);
   for (hipEvent_t e : tev) (void)hipEventDestroy(e);
   for (void *p : device_buffers) (void)hipFree(p);
   for (hipStream_t s : streams) (void)hipStreamDestroy(s);
@@ -469,6 +470,23 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     }
     return SBLAS_OK;
   }
+  if (variant == 6 && k == 1 && streams.empty()) {
+    // the virtual rows' sums (their cut rows merged), then y from them; the
+    // timing covers all three launches
+    hipEvent_t *te = timing_slot();
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
+    if (x32_rows && (st = launch_x_to_f32(n, X, x32_rows, s)) != SBLAS_OK) return st;
+    st = launch_csr_stream(val_type, vsl, d_vblocks, seg.vrp, seg.vcol, seg.vval, X, seg.yv, 1.0,
+                           0.0, d_vpartials, s, x32_rows);
+    if (st == SBLAS_OK && vgroups > 0)
+      st = launch_merge(d_vmrows, d_vmptr, d_vmslots, vgroups, d_vpartials, 1, seg.yv, 1.0, 0.0,
+                        s);
+    if (st == SBLAS_OK) st = launch_seg_merge(seg, alpha, beta, Y, s);
+    if (st != SBLAS_OK) return st;
+    if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
+    launches += 2 + (vgroups > 0 ? 1 : 0);
+    return SBLAS_OK;
+  }
   if (variant == 4 && k == 1 && streams.empty()) {
     hipEvent_t *te = timing_slot();
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
@@ -661,6 +679,73 @@ void Plan::drop_csr5(size_t mark) {
   c5_empty = 0;
 }
 
+// Row blocks over P column segments (variant 6): the segment format, row
+// blocks over its virtual rows (the row-block form's elements per thread, a
+// sub-wave from their mean length) and the merge lists of the virtual rows
+// those blocks cut; every buffer belongs to the format (seg.buffers).
+int Plan::setup_segments(int P) {
+  std::vector<int> h_vrp;
+  int st = seg_build(val_type, m, n, nnz, a0, a1, val, P, seg, h_vrp);
+  if (st != SBLAS_OK) return st;
+  auto own = [&](const void *src, size_t bytes) -> void * {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+    seg.buffers.push_back(p);
+    seg.bytes += static_cast<int64_t>(std::max<size_t>(bytes, 8));
+    if (src && bytes && hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+      return nullptr;
+    return p;
+  };
+  const std::vector<ExecChunk> one{ExecChunk{0, nnz, 0, seg.M}};
+  const int items = sl.items > 0 ? sl.items : 8;
+  const std::vector<RowBlock> vb = build_row_blocks(one, h_vrp.data(), kStreamBlock * items);
+  vsl.items = items;
+  vsl.subwave = pow2_round(static_cast<double>(nnz) / std::max(seg.M, 1) / 4.0, 1, 64);
+  vsl.nblocks = static_cast<int>(vb.size());
+  vsl.xcd_remap = 1;
+  vsl.grid = ((vsl.nblocks + 7) / 8) * 8;
+  std::vector<ExecChunk> as_chunks;
+  as_chunks.reserve(vb.size());
+  for (const RowBlock &r : vb) as_chunks.push_back(ExecChunk{r.lo, r.hi, r.row_begin, r.row_end});
+  std::vector<int> mrows, mptr, mslots;
+  merge_lists(as_chunks, h_vrp.data(), mrows, mptr, mslots);
+  vgroups = static_cast<int>(mrows.size());
+  d_vblocks = static_cast<RowBlock *>(own(vb.data(), sizeof(RowBlock) * vb.size()));
+  d_vmrows = static_cast<int *>(own(mrows.data(), sizeof(int) * mrows.size()));
+  d_vmptr = static_cast<int *>(own(mptr.data(), sizeof(int) * mptr.size()));
+  d_vmslots = static_cast<int *>(own(mslots.data(), sizeof(int) * mslots.size()));
+  d_vpartials = static_cast<double *>(own(nullptr, sizeof(double) * 2 * std::max<size_t>(vb.size(), 1)));
+  workspace_bytes += seg.bytes;
+  if (!d_vblocks || !d_vmrows || !d_vmptr || !d_vmslots || !d_vpartials) {
+    set_error(SBLAS_ERR_ALLOC, "plan: column segment buffers");
+    return SBLAS_ERR_ALLOC;
+  }
+  return SBLAS_OK;
+}
+
+void Plan::drop_segments() {
+  for (void *p : seg.buffers) (void)hipFree(p);
+  workspace_bytes -= seg.bytes;
+  seg = SegmentFormat{};
+  d_vblocks = nullptr;
+  d_vmrows = d_vmptr = d_vmslots = nullptr;
+  d_vpartials = nullptr;
+  vgroups = 0;
+  vsl = StreamLaunch{};
+}
+
+// Column segments for row blocks over x: about 4 MB of FP64 x each, an
+// XCD's L2 (SBLAS_SEGMENTS overrides), at least 2.  R-MAT scale 22 (32 MB of
+// x), all three launches: 8 segments 0.437 ms, 16 0.446 ms, 32 0.581 ms (their
+// virtual rows average 1.6 elements a lane), against 0.488 ms unsegmented.
+int segments_for(int n) {
+  const int env = env_int("SBLAS_SEGMENTS", 0);
+  if (env > 0) return std::min(env, kSegmentsMax);
+  const int64_t bytes = static_cast<int64_t>(n) * 8;
+  return static_cast<int>(std::min<int64_t>(kSegmentsMax,
+                                            std::max<int64_t>(2, (bytes + (4 << 20) - 1) >> 22)));
+}
+
 // Times forms (kernel, variant) of the plan's own matrix with x = 0 and a
 // scratch y: launches cycle through the forms so a clock ramp affects all
 // alike, untimed for at least 20 ms, then five timed rounds; ms[i] is form
@@ -770,8 +855,8 @@ int Plan::finish_csr(const std::vector<int> &h_rp) {
     nxo = env_int("SBLAS_NARROW_X", 0);
     if (nxo <= 0) nxo = nnz >= (int64_t{1} << 20) ? 2 : 1;
   }
-  if (variant == 2 && kernel == SBLAS_KERNEL_VECTOR && val_type != SBLAS_VAL_F64 && nxo == 2 &&
-      n > 0) {
+  if ((variant == 2 || variant == 6) && kernel == SBLAS_KERNEL_VECTOR &&
+      val_type != SBLAS_VAL_F64 && nxo == 2 && n > 0) {
     x32_rows = static_cast<float *>(dev_alloc(sizeof(float) * static_cast<size_t>(n)));
     if (!x32_rows) {
       set_error(SBLAS_ERR_ALLOC, "plan: FP32 x copy allocation failed");
@@ -792,7 +877,9 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
   }
   // kernel 1 form: row blocks staged in LDS unless the sub-wave form is asked
   // for; the sub-wave form also serves multi-RHS blocks and per-chunk streams
-  variant = (opts.variant == 1 || opts.streams > 1) ? 1 : (opts.variant == 4 ? 4 : 2);
+  variant = (opts.variant == 1 || opts.streams > 1) ? 1
+            : (opts.variant == 4 || opts.variant == 6) ? opts.variant
+                                                        : 2;
   if (opts.variant == 0) variant = env_int("SBLAS_VARIANT", variant);
   int st;
   if ((st = build_csr_chunks(h_rp.data())) != SBLAS_OK) return st;
@@ -875,13 +962,19 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
       return setup_stream(h_rp.data());  // row blocks serve multi-RHS products
     }
   }
-  if (variant != 2 && variant != 4) return SBLAS_OK;
-  const bool want_sorted = variant == 4;
+  if (variant != 2 && variant != 4 && variant != 6) return SBLAS_OK;
+  const int asked = variant;
   variant = 2;
   if ((st = setup_stream(h_rp.data())) != SBLAS_OK) return st;  // also serves multi-RHS
-  if (want_sorted) {
+  if (asked == 4) {
     variant = 4;
     return setup_sorted(h_rp.data());
+  }
+  if (asked == 6) {
+    if (nnz == 0) return SBLAS_OK;  // nothing to segment: row blocks
+    if ((st = setup_segments(segments_for(n))) != SBLAS_OK) return st;
+    variant = 6;
+    return SBLAS_OK;
   }
   // kernel AUTO, no column panels: row blocks, column-sorted row blocks or
   // the CSR5-style form
@@ -900,7 +993,7 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
   }
   if (cached) {
     tuned = 3;
-    for (int i = 0; i < 3; i++) trial_ms[i] = found.ms[i];
+    for (int i = 0; i < 4; i++) trial_ms[i] = found.ms[i];
     if (found.kernel == SBLAS_KERNEL_CSR5) {
       kernel = SBLAS_KERNEL_CSR5;
       return setup_csr5(h_rp);
@@ -908,6 +1001,11 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
     if (found.variant == 4) {
       variant = 4;
       return setup_sorted(h_rp.data());
+    }
+    if (found.variant == 6) {
+      if ((st = setup_segments(segments_for(n))) != SBLAS_OK) return st;
+      variant = 6;
+      return SBLAS_OK;
     }
     return SBLAS_OK;
   }
@@ -917,8 +1015,20 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
   if ((st = setup_sorted(h_rp.data())) != SBLAS_OK) return st;
   const size_t mark = device_buffers.size();
   if ((st = setup_csr5(h_rp)) != SBLAS_OK) return st;
-  const std::vector<std::pair<int, int>> forms{
+  std::vector<std::pair<int, int>> forms{
       {SBLAS_KERNEL_VECTOR, 2}, {SBLAS_KERNEL_CSR5, 2}, {SBLAS_KERNEL_VECTOR, 4}};
+  // column segments where x is several times an XCD's L2 (4 MB)
+  if (static_cast<int64_t>(n) * 8 >= (int64_t{16} << 20)) {
+    st = setup_segments(segments_for(n));
+    if (st == SBLAS_ERR_ALLOC || st == SBLAS_ERR_UNSUPPORTED) {
+      clear_error();
+      drop_segments();
+    } else if (st != SBLAS_OK) {
+      return st;
+    } else {
+      forms.push_back({SBLAS_KERNEL_VECTOR, 6});
+    }
+  }
   std::vector<float> ms;
   if ((st = trial(forms, ms)) != SBLAS_OK) return st;
   tuned = 2;
@@ -931,8 +1041,9 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
   variant = forms[best].second;
   if (kernel != SBLAS_KERNEL_CSR5) drop_csr5(mark);
   if (variant != 4) drop_sorted();
+  if (variant != 6 && !seg.buffers.empty()) drop_segments();
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tune_cache[key] = TuneResult{kernel, variant, {ms[0], ms[1], ms[2]}};
+  g_tune_cache[key] = TuneResult{kernel, variant, {trial_ms[0], trial_ms[1], trial_ms[2], trial_ms[3]}};
   return SBLAS_OK;
 }
 
@@ -943,6 +1054,7 @@ bool Plan::uses_tiles(int k) const {
 
 bool Plan::has_merge(int k) const {
   if (uses_tiles(k)) return tiles > 0;
+  if (variant == 6 && k == 1 && streams.empty()) return false;  // merges run inside the product
   if (variant == 3 && k == 1 && streams.empty()) return pgroups > 0;
   if (variant == 4 && k == 1 && streams.empty()) return sgroups > 0;
   if (variant >= 2 && streams.empty()) return bgroups > 0;  // row blocks (any k)
@@ -950,6 +1062,7 @@ bool Plan::has_merge(int k) const {
 }
 
 int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) {
+  if (variant == 6 && k == 1 && streams.empty()) return SBLAS_OK;
   if (uses_tiles(k))
     return tiles > 0 ? launch_tile_fixup(carry_count(), d_carry, Y, alpha, beta, s) : SBLAS_OK;
   if (variant == 3 && k == 1 && streams.empty())
@@ -1277,6 +1390,13 @@ int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info) {
   info->narrow_x = (p.variant == 3 && p.panel.x32) || p.x32_rows ? 1 : 0;
   info->panel_elements = p.variant == 3 ? p.panel.padded : 0;
   if (p.variant == 2) info->subwave = p.sl.subwave;
+  if (p.variant == 6) {
+    info->subwave = p.vsl.subwave;
+    info->row_blocks = p.vsl.nblocks;
+  }
+  info->segments = p.variant == 6 ? p.seg.P : 0;
+  info->virtual_rows = p.variant == 6 ? p.seg.M : 0;
+  info->trial_seg_ms = p.trial_ms[3];
   return SBLAS_OK;
 }
 

@@ -85,6 +85,16 @@ struct Plan {
   int *d_smrows = nullptr, *d_smptr = nullptr, *d_smslots = nullptr;
   int sgroups = 0;
 
+  // row blocks over column segments of kernel 1 (variant 6): the format, the
+  // row blocks over its virtual rows and the merge lists of the virtual rows
+  // their windows cut
+  SegmentFormat seg;
+  StreamLaunch vsl{};
+  RowBlock *d_vblocks = nullptr;
+  int *d_vmrows = nullptr, *d_vmptr = nullptr, *d_vmslots = nullptr;
+  int vgroups = 0;
+  double *d_vpartials = nullptr;
+
   // CSR5-style format (kernel 3)
   uint32_t *d_c5_tile_nz = nullptr, *d_c5_flags = nullptr;
   int *d_c5_rows_nz = nullptr, *d_c5_empty = nullptr;
@@ -104,7 +114,8 @@ struct Plan {
   // how the kernel was chosen: 0 asked for, 1 rule, 2 timed trial, 3 cached
   // trial; the trial's median times (row blocks, CSR5-style) in ms
   int tuned = 0;
-  float trial_ms[3] = {0.f, 0.f, 0.f};  // row blocks, CSR5-style, column-sorted row blocks
+  // row blocks, CSR5-style, column-sorted row blocks, column segments
+  float trial_ms[4] = {0.f, 0.f, 0.f, 0.f};
   // FP32 copy of x for row blocks over narrow values (narrow_x = 2); nullptr: FP64 x
   float *x32_rows = nullptr;
   int launches = 0;
@@ -134,6 +145,8 @@ struct Plan {
   int setup_sorted(const int *h_rp);
   void drop_sorted();
   void drop_csr5(size_t mark);
+  int setup_segments(int P);
+  void drop_segments();
   // times the given (kernel, variant) forms on the plan's own matrix; ms[i]
   // receives the median of form i
   int trial(const std::vector<std::pair<int, int>> &forms, std::vector<float> &ms);

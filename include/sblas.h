@@ -209,7 +209,10 @@ typedef struct sblas_opts {
   int variant;     /* kernel 1 form: 0 auto, 1 sub-wave per row, 2 row blocks */
                    /* staged in LDS, 3 column panels with x staged in LDS,   */
                    /* 4 row blocks stored sorted by column (x read in column */
-                   /* order, products put back in element order in LDS)      */
+                   /* order, products put back in element order in LDS), 6  */
+                   /* row blocks over column segments (each row cut at      */
+                   /* column-segment edges, about 2 MB of x per segment; the */
+                   /* pieces summed segment by segment, then merged per row) */
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
   int panel_rows;  /* column-panel form: rows per row panel (1..2048, or   */
                    /* 1..1536 in a batch; 0 auto)                           */
@@ -279,7 +282,8 @@ typedef struct sblas_plan_info {
   int64_t max_row;
   int launches;      /* kernel launches per sblas_plan_spmv                 */
   int variant;       /* kernel 1 form in use (1 sub-wave, 2 row blocks,     */
-                     /* 3 column panels)                                    */
+                     /* 3 column panels, 4 column-sorted row blocks, 6     */
+                     /* row blocks over column segments)                   */
   int row_blocks;    /* row blocks of the row-block form                    */
   int items;         /* elements per thread of the row-block form           */
   int panel_rows;    /* column-panel form: rows per row panel               */
@@ -294,6 +298,10 @@ typedef struct sblas_plan_info {
   int narrow_x;      /* column-panel form: 1 when x is staged in FP32       */
   int64_t panel_elements; /* column-panel form: elements stored (padding    */
                           /* included)                                      */
+  int segments;           /* variant 6: column segments                     */
+  int virtual_rows;       /* variant 6: (segment, row) pieces with elements */
+  float trial_seg_ms;     /* trial median of variant 6 (0: not tried)       */
+  int reserved_info;
 } sblas_plan_info;
 int sblas_plan_get_info(sblas_plan plan, sblas_plan_info *info);
 /* Chunk descriptors of the plan (reference rule of opts.part_mode). */

@@ -1,0 +1,138 @@
+"""Row blocks over column segments (kernel 1, variant 6) against the oracle:
+the elements of each row are cut at the segment edges into virtual rows,
+summed segment by segment and merged per row.  Covered: segment counts that
+do not divide n, empty rows and empty segments, rows long enough that their
+pieces are cut across row blocks (partials merged before the per-row merge),
+alpha / beta != 0, FP64 / FP32 / BF16 values, multi-RHS products (which run
+over the unsegmented row blocks), an empty matrix, and the form chosen by
+AUTO's timed trial when x is four times an XCD's L2 or larger."""
+import numpy as np
+import pytest
+
+from conftest import random_csr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _plan(A_rp, A_col, vals, m, n, P, monkeypatch, **opts):
+    import torch
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    monkeypatch.setenv("SBLAS_SEGMENTS", str(P))
+    rp, col = torch.from_numpy(A_rp).cuda(), torch.from_numpy(A_col).cuda()
+    return SpmvPlan("csr", m, n, rp, col, torch.from_numpy(vals).cuda(), kernel="vector",
+                    variant=6, **opts)
+
+
+@pytest.mark.parametrize("P", [2, 5, 16])
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+def test_segments_parity(oracle, monkeypatch, P, vt):
+    import torch
+    from hpda_sblas_pnnl_amd import gen as G
+    rng = np.random.default_rng(300 + P)
+    m, n = 6000, 5003
+    rp, col, val = random_csr(rng, m, n, 20, empty_frac=0.25, long_rows=3, long_len=5000)
+    if vt == "f64":
+        dv, ref_fn = val, lambda x, y0: oracle.csr_spmv(m, n, val, rp, col, x, 1.25, -0.5, y0)
+    elif vt == "f32":
+        v32 = val.astype(np.float32)
+        dv = v32
+        ref_fn = lambda x, y0: oracle.csr_spmv_f32val(m, v32, rp, col, x, 1.25, -0.5, y0)
+    else:
+        bits = G.to_bf16_bits(val)
+        dv = bits.view(np.int16)
+        ref_fn = lambda x, y0: oracle.csr_spmv_bf16val(m, bits, rp, col, x, 1.25, -0.5, y0)
+    plan = _plan(rp, col, dv, m, n, P, monkeypatch)
+    info = plan.info()
+    assert info["variant"] == 6 and info["segments"] == P
+    # every (segment, row) piece that holds elements is a virtual row
+    seg = (col.astype(np.int64) * P) // n
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    assert info["virtual_rows"] == len(np.unique(seg * m + rows))
+    for rep in range(2):
+        x = rng.uniform(-1, 1, n)
+        y0 = rng.uniform(-1, 1, m)
+        dy = torch.from_numpy(y0.copy()).cuda()
+        plan.spmv(torch.from_numpy(x).cuda(), dy, 1.25, -0.5)
+        torch.cuda.synchronize()
+        assert oracle.err_normwise(dy.cpu().numpy(), ref_fn(x, y0)) <= TOL, rep
+
+
+def test_segments_long_rows_cut_across_blocks(oracle, monkeypatch):
+    """Rows of 18000 elements in two segments: pieces of 9000 elements, cut
+    across 2048-element row blocks; beta = 0 with y holding NaN first."""
+    import torch
+    n = 20000
+    r, c, v = oracle.gen_g(n)
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, r + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    rng = np.random.default_rng(5)
+    vals = rng.uniform(-1, 1, len(c))
+    plan = _plan(rp, c, vals, n, n, 2, monkeypatch)
+    x = rng.uniform(-1, 1, n)
+    dy = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+    plan.spmv(torch.from_numpy(x).cuda(), dy, 1.0, 0.0)
+    torch.cuda.synchronize()
+    ref = oracle.csr_spmv(n, n, vals, rp, c, x, 1.0, 0.0, np.zeros(n))
+    assert oracle.err_normwise(dy.cpu().numpy(), ref) <= TOL
+    assert plan.info()["launches"] == 3  # row blocks, the cut pieces' merge, the per-row merge
+
+
+def test_segments_multi_rhs_and_empty(oracle, monkeypatch):
+    import torch
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    rng = np.random.default_rng(9)
+    m, n = 3000, 4000
+    rp, col, val = random_csr(rng, m, n, 15, empty_frac=0.3)
+    plan = _plan(rp, col, val, m, n, 4, monkeypatch)
+    K = 8
+    X = rng.uniform(-1, 1, (n, K))
+    Y0 = rng.uniform(-1, 1, (m, K))
+    dY = torch.from_numpy(Y0.copy()).cuda()
+    plan.spmm(torch.from_numpy(X).cuda(), dY, 0.5, 2.0)
+    torch.cuda.synchronize()
+    ref = oracle.csr_spmm(m, K, val, rp, col, X, 0.5, 2.0, Y0)
+    assert oracle.err_normwise(dY.cpu().numpy().ravel(), ref.ravel()) <= TOL
+    # no elements: the form falls back to row blocks, y = beta*y
+    monkeypatch.setenv("SBLAS_SEGMENTS", "4")
+    rp0 = np.zeros(101, np.int32)
+    p0 = SpmvPlan("csr", 100, 50, torch.from_numpy(rp0).cuda(),
+                  torch.zeros(0, dtype=torch.int32, device="cuda"),
+                  torch.zeros(0, dtype=torch.float64, device="cuda"), kernel="vector", variant=6)
+    assert p0.info()["variant"] == 2
+    y0 = rng.uniform(-1, 1, 100)
+    dy = torch.from_numpy(y0.copy()).cuda()
+    p0.spmv(torch.zeros(50, dtype=torch.float64, device="cuda"), dy, 1.0, 0.25)
+    torch.cuda.synchronize()
+    assert np.array_equal(dy.cpu().numpy(), 0.25 * y0)
+
+
+def test_auto_trial_includes_segments(oracle, monkeypatch):
+    """R-MAT scale 21 (x 16 MB): the timed trial covers column segments and
+    keeps the fastest of the four forms."""
+    import torch
+    from hpda_sblas_pnnl_amd import gen as G
+    from hpda_sblas_pnnl_amd.device import SpmvPlan
+    monkeypatch.delenv("SBLAS_AUTOTUNE", raising=False)
+    monkeypatch.delenv("SBLAS_SEGMENTS", raising=False)
+    A = G.rmat_csr(scale=21, edge_factor=8, seed=17)
+    rp, col, val = (torch.from_numpy(a).cuda() for a in (A.row_ptr, A.col, A.val))
+    plan = SpmvPlan("csr", A.m, A.n, rp, col, val)
+    info = plan.info()
+    if info["tuned"] == 2:  # a trial in this plan (not a decision cached earlier)
+        rows_ms, csr5_ms, sorted_ms = info["trial_ms"]
+        seg_ms = info["trial_seg_ms"]
+        assert min(rows_ms, csr5_ms, sorted_ms, seg_ms) > 0
+        best = min(rows_ms, csr5_ms, sorted_ms, seg_ms)
+        if best == seg_ms:
+            assert info["kernel"] == 1 and info["variant"] == 6 and info["segments"] == 4
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-1, 1, A.n)
+    y0 = rng.uniform(-1, 1, A.m)
+    dy = torch.from_numpy(y0.copy()).cuda()
+    plan.spmv(torch.from_numpy(x).cuda(), dy, 1.0, 0.5)
+    torch.cuda.synchronize()
+    ref = oracle.csr_spmv(A.m, A.n, A.val, A.row_ptr, A.col, x, 1.0, 0.5, y0)
+    assert oracle.err_normwise(dy.cpu().numpy(), ref) <= TOL
