@@ -267,6 +267,7 @@ def extras_rmat(log_to):
     log_to(f"[bench] rmat cpu baseline: {out['cpu_baseline']}")
 
     def run(plan, fmt_bytes_val=8, fmt="csr"):
+        info = plan.info()
         dy = dy0.clone()
         plan.spmv(dx, dy, alpha, beta)
         torch.cuda.synchronize()
@@ -279,11 +280,13 @@ def extras_rmat(log_to):
             cb += A.nnz * 4 - (A.m + 1) * 4
         return {"ms": round(ms, 4), "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
                 "GBs": round(cb / ms / 1e6, 1), "frac": round(cb / ms / 1e6 / PEAK_HBM_GBS, 4),
-                "err_normwise": err, "err_maxrow": maxrow}
+                "err_normwise": err, "err_maxrow": maxrow, "kernel": info["kernel"],
+                "variant": info["variant"]}
 
     kernels = {}
-    for name, kw in (("kernel1_rowblock", {"kernel": "vector"}), ("kernel2_merge", {"kernel": "merge"}),
-                     ("kernel3_csr5", {"kernel": "csr5"})):
+    for name, kw in (("auto", {}), ("kernel1_rowblock", {"kernel": "vector"}),
+                     ("kernel1_segments", {"kernel": "vector", "variant": 6}),
+                     ("kernel2_merge", {"kernel": "merge"}), ("kernel3_csr5", {"kernel": "csr5"})):
         p = SpmvPlan("csr", A.m, A.n, rp, col, val, **kw)
         kernels[name] = run(p)
         del p
@@ -298,7 +301,7 @@ def extras_rmat(log_to):
     kernels["csc_transposed"] = run(p)
     del p, dcp, drows, dcv
     out["fp64"] = kernels
-    base_ms = kernels["kernel1_rowblock"]["ms"]
+    base_ms = kernels["auto"]["ms"]  # narrow-value plans below are AUTO plans too
     out["ingest"] = extras_ingest(A, rp, col, val, log_to)
 
     prec = {}
@@ -625,7 +628,8 @@ def main():
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--chunks", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0,
-                    help="kernel 1 form: 1 sub-wave, 2 row blocks, 3 column panels, 4 sorted blocks")
+                    help="kernel 1 form: 1 sub-wave, 2 row blocks, 3 column panels, 4 sorted "
+                         "blocks, 6 row blocks over column segments")
     ap.add_argument("--values", default="f64", choices=["f64", "f32", "bf16"],
                     help="value type of the timed plan (profiling the narrow forms)")
     ap.add_argument("--items", type=int, default=0)

@@ -106,6 +106,7 @@ int launch_x_to_f32(int n, const double *x, float *x32, hipStream_t s);
 // row (s, r); virtual rows holding elements are numbered segment-major, rows
 // ascending, and the matrix is copied in that order (vrp, vcol, vval).
 constexpr int kSegmentsMax = 64;
+constexpr int kSegMergeRows = 256;  // real rows per workgroup of the per-row merge
 struct SegmentFormat {
   int P = 0;                    // column segments
   int m = 0;                    // real rows
@@ -113,8 +114,9 @@ struct SegmentFormat {
   int *vrp = nullptr;           // [M + 1] virtual row pointers
   int *vcol = nullptr;          // [nnz] columns in virtual-row order
   void *vval = nullptr;         // [nnz] values in virtual-row order
-  int *vptr = nullptr;          // [m + 1] real row r's virtual rows: vidx[vptr[r]..vptr[r+1])
-  int *vidx = nullptr;          // [M] their indices, segments ascending
+  int *vrow = nullptr;          // [M] real row of each virtual row
+  int *vtab = nullptr;          // [(ceil(m/kSegMergeRows) + 1) * P] first virtual row of
+                                // segment s at or after row b*kSegMergeRows: [b*P + s]
   double *yv = nullptr;         // [M] one sum per virtual row
   std::vector<void *> buffers;
   int64_t bytes = 0;
