@@ -292,9 +292,12 @@ def extras_rmat(log_to):
         del p
         log_to(f"[bench] rmat {name}: {kernels[name]}")
     ri = torch.from_numpy(G.coo_rows(A)).to(dev)
-    p = SpmvPlan("coo", A.m, A.n, ri, col, val)
-    kernels["coo_segmented"] = run(p, fmt="coo")
-    del p, ri
+    for name, kw in (("coo_auto", {}), ("coo_segmented", {"kernel": "coo"})):
+        # AUTO on COO builds row pointers and runs the CSR chooser
+        p = SpmvPlan("coo", A.m, A.n, ri, col, val, **kw)
+        kernels[name] = run(p, fmt="coo")
+        del p
+    del ri
     cp, rows, cv = G.to_csc(A)
     dcp, drows, dcv = (torch.from_numpy(v).to(dev) for v in (cp, rows, cv))
     p = SpmvPlan("csc", A.m, A.n, dcp, drows, dcv)

@@ -736,8 +736,9 @@ void Plan::drop_segments() {
 
 // Column segments for row blocks over x: about 4 MB of FP64 x each, an
 // XCD's L2 (SBLAS_SEGMENTS overrides), at least 2.  R-MAT scale 22 (32 MB of
-// x), all three launches: 8 segments 0.437 ms, 16 0.446 ms, 32 0.581 ms (their
-// virtual rows average 1.6 elements a lane), against 0.488 ms unsegmented.
+// x), all three launches: 8 segments 0.423 ms (FP32 values 0.370), 16 segments
+// 0.418 ms (0.377), against 0.491 ms (0.419) unsegmented; 32 segments ran
+// 0.58 ms, their virtual rows averaging 1.6 elements a lane.
 int segments_for(int n) {
   const int env = env_int("SBLAS_SEGMENTS", 0);
   if (env > 0) return std::min(env, kSegmentsMax);

@@ -1,6 +1,7 @@
 """The autotuner behind kernel AUTO, on the device: column panels for a
-banded matrix (structural rule), a timed trial between row blocks and the
-CSR5-style kernel for an R-MAT matrix (the faster one is kept, and a second
+banded matrix (structural rule), a timed trial over row blocks, column-sorted
+row blocks and the CSR5-style kernel for an R-MAT matrix whose x is under
+16 MB, so without column segments (the fastest one is kept, and a second
 plan of the same row profile reuses the decision), row blocks by rule for a
 matrix below the trial size.  Every AUTO plan stays within 1e-12 of the
 oracle."""
@@ -43,6 +44,7 @@ def test_auto_rmat_trial_and_cache(oracle, monkeypatch):
     assert first["tuned"] in (2, 3)  # a trial, or one cached earlier in this process
     rows_ms, csr5_ms, sorted_ms = first["trial_ms"]
     assert rows_ms > 0 and csr5_ms > 0 and sorted_ms > 0
+    assert first["trial_seg_ms"] == 0 and first["segments"] == 0
     best = min(rows_ms, csr5_ms, sorted_ms)
     if best == csr5_ms:
         assert first["kernel"] == 3
