@@ -114,7 +114,8 @@ struct SegmentFormat {
   int *vrp = nullptr;           // [M + 1] virtual row pointers
   int *vcol = nullptr;          // [nnz] columns in virtual-row order
   void *vval = nullptr;         // [nnz] values in virtual-row order
-  int *vrow = nullptr;          // [M] real row of each virtual row
+  uint8_t *vrow = nullptr;      // [M] real row of each virtual row, less its merge
+                                // block's first row (row mod kSegMergeRows)
   int *vtab = nullptr;          // [(ceil(m/kSegMergeRows) + 1) * P] first virtual row of
                                 // segment s at or after row b*kSegMergeRows: [b*P + s]
   double *yv = nullptr;         // [M] one sum per virtual row

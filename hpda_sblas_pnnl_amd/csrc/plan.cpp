@@ -700,7 +700,9 @@ int Plan::setup_segments(int P) {
   const int items = sl.items > 0 ? sl.items : 8;
   const std::vector<RowBlock> vb = build_row_blocks(one, h_vrp.data(), kStreamBlock * items);
   vsl.items = items;
-  vsl.subwave = pow2_round(static_cast<double>(nnz) / std::max(seg.M, 1) / 4.0, 1, 64);
+  vsl.subwave = opts.subwave > 0
+                    ? opts.subwave
+                    : pow2_round(static_cast<double>(nnz) / std::max(seg.M, 1) / 4.0, 1, 64);
   vsl.nblocks = static_cast<int>(vb.size());
   vsl.xcd_remap = 1;
   vsl.grid = ((vsl.nblocks + 7) / 8) * 8;
@@ -734,17 +736,17 @@ void Plan::drop_segments() {
   vsl = StreamLaunch{};
 }
 
-// Column segments for row blocks over x: about 4 MB of FP64 x each, an
-// XCD's L2 (SBLAS_SEGMENTS overrides), at least 2.  R-MAT scale 22 (32 MB of
-// x), all three launches: 8 segments 0.423 ms (FP32 values 0.370), 16 segments
-// 0.418 ms (0.377), against 0.491 ms (0.419) unsegmented; 32 segments ran
-// 0.58 ms, their virtual rows averaging 1.6 elements a lane.
+// Column segments for row blocks over x: about 2 MB of FP64 x each, half an
+// XCD's L2, so the segment stays there beside the streamed elements
+// (SBLAS_SEGMENTS overrides), at least 2.  R-MAT scale 22 (32 MB of x), all
+// three launches, against 0.491 ms unsegmented (FP32 values 0.419): 8 segments
+// 0.416 ms (0.364), 12 0.411, 16 0.401 (0.355), 24 0.404, 32 0.411.
 int segments_for(int n) {
   const int env = env_int("SBLAS_SEGMENTS", 0);
   if (env > 0) return std::min(env, kSegmentsMax);
   const int64_t bytes = static_cast<int64_t>(n) * 8;
   return static_cast<int>(std::min<int64_t>(kSegmentsMax,
-                                            std::max<int64_t>(2, (bytes + (4 << 20) - 1) >> 22)));
+                                            std::max<int64_t>(2, (bytes + (2 << 20) - 1) >> 21)));
 }
 
 // Times forms (kernel, variant) of the plan's own matrix with x = 0 and a
@@ -1004,7 +1006,13 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
       return setup_sorted(h_rp.data());
     }
     if (found.variant == 6) {
-      if ((st = setup_segments(segments_for(n))) != SBLAS_OK) return st;
+      st = setup_segments(segments_for(n));
+      if (st == SBLAS_ERR_ALLOC || st == SBLAS_ERR_UNSUPPORTED) {  // row blocks instead
+        clear_error();
+        drop_segments();
+        return SBLAS_OK;
+      }
+      if (st != SBLAS_OK) return st;
       variant = 6;
       return SBLAS_OK;
     }

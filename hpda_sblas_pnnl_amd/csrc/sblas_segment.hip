@@ -7,18 +7,20 @@
 // XCD's 4 MB L2 and a row block's gathers spread over all of it: a wave's
 // gather waits for its slowest lane, and nearly every one has lanes that miss.
 // The columns are cut into P segments of equal width (n/P entries of x, about
-// 4 MB of FP64 each).  The elements of row r in segment s (a contiguous run, columns
-// being sorted within rows) form a virtual row (s, r); the virtual rows that
-// hold elements are numbered segment by segment, rows ascending, and the
-// matrix is copied in that order.  The row-block kernel runs over the virtual
-// rows as over any CSR matrix, writing one sum per virtual row; with the XCD
-// remap each XCD works through a contiguous run of them, so through one or
-// two segments of x at a time, which stay in its L2.  A merge then forms
+// 2 MB of FP64 each).  The elements of row r in segment s (a contiguous run,
+// columns being sorted within rows) form a virtual row (s, r); the virtual
+// rows that hold elements are numbered segment by segment, rows ascending, and
+// the matrix is copied in that order.  The row-block kernel runs over the
+// virtual rows as over any CSR matrix, writing one sum per virtual row; with
+// the XCD remap each XCD works through a contiguous run of them, so through a
+// few segments of x, one after another, each held in its L2 while it is read.
+// A merge then forms
 // y[r] = alpha * (sum over its virtual rows, segments ascending) + beta*y[r],
 // each 256-row block of y reading every segment's contiguous run of sums.
 //
-// On R-MAT scale 22 (P = 8, 6.75 M virtual rows) L2 misses fall from 22.8 M
-// to 8.2 M per product and reads from HBM from 2.9 to 1.0 GB.
+// On R-MAT scale 22 at P = 8 (6.75 M virtual rows) L2 misses fall from 22.8 M
+// to 8.2 M per product and reads from HBM from 2.9 to 1.0 GB; P = 16 (9.3 M
+// virtual rows) runs faster still.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -63,15 +65,18 @@ __global__ void k_seg_flags(int64_t count, const int *__restrict__ cnt, int *__r
   if (i < count) flag[i] = cnt[i] > 0;
 }
 
+static_assert(kSegMergeRows <= 256, "row offsets in a merge block are 8-bit");
+
 // virtual row v of every (s, r) that holds elements: its first element
-// (voff, the exclusive scan of cnt in segment-major order) and its real row
+// (voff, the exclusive scan of cnt in segment-major order) and its real row's
+// place in its merge block
 __global__ void k_seg_rows(int64_t count, int m, const int *__restrict__ cnt,
                            const int *__restrict__ vindex, const int *__restrict__ voff,
-                           int *__restrict__ vrp, int *__restrict__ vrow) {
+                           int *__restrict__ vrp, uint8_t *__restrict__ vrow) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < count && cnt[i] > 0) {
     vrp[vindex[i]] = voff[i];
-    vrow[vindex[i]] = static_cast<int>(i % m);
+    vrow[vindex[i]] = static_cast<uint8_t>((i % m) % kSegMergeRows);
   }
 }
 
@@ -112,25 +117,80 @@ __global__ void k_seg_tab(int nb, int m, int P, const int *__restrict__ vindex,
   vtab[i] = vindex[static_cast<int64_t>(s) * m + r];
 }
 
-// Rows [b*256, b*256 + 256): for each segment in turn, its virtual rows of
-// these rows (a contiguous run of yv, each real row at most once) add into
-// the rows' sums in LDS; then y[r] = alpha * sum + beta * y[r].  The sums take
-// the segments in ascending order, starting from 0.
+// The per-row merge.  Rows [b*256, b*256 + 256), segments in groups of
+// kSegGroup: the group's virtual rows of these rows (per segment a contiguous
+// run of yv, each real row at most once) are loaded together, as one list
+// over the runs, into an LDS table [segment][row] that is zero where a row has
+// no elements in the segment; then each thread adds its row's entries,
+// segments ascending, to a sum that starts from 0 (adding a zero entry leaves
+// the sum's bits as they are), and writes y[r] = alpha * sum + beta * y[r].
+// Each thread issues the loads of up to kSegUnroll list entries before it
+// waits for any.  The kernel waits on its dependent loads (run bounds, then
+// runs), so it wants many workgroups per CU: at 8 segments a group the table
+// takes 16 KB and eight workgroups fit (on R-MAT scale 22 at 16 segments the
+// whole product ran 0.402 ms; 16 segments a group, 32 KB: 0.429 ms; one
+// barrier per segment over LDS sums: 0.417 ms; persistent workgroups loading
+// the next group's runs during the current one, four per CU: 0.428-0.442 ms).
+constexpr int kSegGroup = 8;
+constexpr int kSegUnroll = 4;
 __global__ __launch_bounds__(kSegMergeRows) void k_seg_merge(
-    int m, int P, const int *__restrict__ vtab, const int *__restrict__ vrow,
+    int m, int P, const int *__restrict__ vtab, const uint8_t *__restrict__ vrow,
     const double *__restrict__ yv, double *__restrict__ y, double alpha, double beta) {
-  __shared__ double acc[kSegMergeRows];
+  __shared__ double tab[kSegGroup][kSegMergeRows];
+  __shared__ int run_lo[kSegGroup], run_len[kSegGroup];
   const int t = threadIdx.x, b = blockIdx.x;
-  const int r0 = b * kSegMergeRows;
-  acc[t] = 0.0;
-  __syncthreads();
-  for (int s = 0; s < P; s++) {
-    const int a = vtab[b * P + s], e = vtab[(b + 1) * P + s];
-    for (int i = a + t; i < e; i += kSegMergeRows) acc[vrow[i] - r0] += yv[i];
-    __syncthreads();  // segment s added before segment s + 1
+  const int r = b * kSegMergeRows + t;
+  const double y_old = (beta != 0.0 && r < m) ? y[r] : 0.0;
+  double acc = 0.0;
+  for (int s0 = 0; s0 < P; s0 += kSegGroup) {
+    const int g = min(kSegGroup, P - s0);
+    if (t < g) {
+      const int lo = vtab[b * P + s0 + t];
+      run_lo[t] = lo;
+      run_len[t] = vtab[(b + 1) * P + s0 + t] - lo;
+    }
+#pragma unroll
+    for (int k = 0; k < kSegGroup; k++) tab[k][t] = 0.0;
+    __syncthreads();
+    int pre[kSegGroup + 1];  // list position where each run starts
+    pre[0] = 0;
+#pragma unroll
+    for (int k = 0; k < kSegGroup; k++) pre[k + 1] = pre[k] + (k < g ? run_len[k] : 0);
+    const int total = pre[kSegGroup];
+    for (int j0 = t; j0 < total; j0 += kSegUnroll * kSegMergeRows) {
+      int seg[kSegUnroll], idx[kSegUnroll];
+#pragma unroll
+      for (int u = 0; u < kSegUnroll; u++) {
+        const int j = j0 + u * kSegMergeRows;  // entries past the list load entry j0's
+        const int jj = j < total ? j : j0;     // place again and store nothing
+        int k = 0, base = 0;                   // static indexes keep pre in registers
+#pragma unroll
+        for (int q = 1; q < kSegGroup; q++)
+          if (jj >= pre[q]) {
+            k = q;
+            base = pre[q];
+          }
+        seg[u] = j < total ? k : -1;
+        idx[u] = run_lo[k] + (jj - base);
+      }
+      uint8_t row[kSegUnroll];
+      double v[kSegUnroll];
+#pragma unroll
+      for (int u = 0; u < kSegUnroll; u++) {
+        row[u] = vrow[idx[u]];
+        v[u] = yv[idx[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kSegUnroll; u++)
+        if (seg[u] >= 0) tab[seg[u]][row[u]] = v[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSegGroup; k++)
+      if (k < g) acc += tab[k][t];
+    __syncthreads();  // the table is read before the next group clears it
   }
-  const int r = r0 + t;
-  if (r < m) y[r] = (beta == 0.0) ? alpha * acc[t] : alpha * acc[t] + beta * y[r];
+  if (r < m) y[r] = (beta == 0.0) ? alpha * acc : alpha * acc + beta * y_old;
 }
 
 int exclusive_sum(const int *in, int *out, int64_t count, hipStream_t s) {
@@ -164,6 +224,10 @@ int seg_build(int val_type, int m, int n, int64_t nnz, const int *rp, const int 
     return SBLAS_ERR_ARGS;
   }
   const int64_t pairs = static_cast<int64_t>(P) * m;
+  if (pairs + 1 > INT32_MAX) {  // the scans and virtual row numbers are int
+    set_error(SBLAS_ERR_UNSUPPORTED, "column segments: too many (segment, row) pairs");
+    return SBLAS_ERR_UNSUPPORTED;
+  }
   std::vector<void *> temps;
   auto tmp = [&](size_t bytes) -> void * {
     void *p = nullptr;
@@ -229,7 +293,7 @@ int seg_build(int val_type, int m, int n, int64_t nnz, const int *rp, const int 
   f.vcol = static_cast<int *>(alloc(sizeof(int) * static_cast<size_t>(nnz)));
   f.vval = alloc(vb * static_cast<size_t>(nnz));
   const int nb = (m + kSegMergeRows - 1) / kSegMergeRows;
-  f.vrow = static_cast<int *>(alloc(sizeof(int) * static_cast<size_t>(M)));
+  f.vrow = static_cast<uint8_t *>(alloc(static_cast<size_t>(M)));
   f.vtab = static_cast<int *>(alloc(sizeof(int) * (static_cast<size_t>(nb) + 1) * P));
   f.yv = static_cast<double *>(alloc(sizeof(double) * static_cast<size_t>(M)));
   if (!f.vrp || !f.vcol || !f.vval || !f.vrow || !f.vtab || !f.yv) {

@@ -211,7 +211,7 @@ typedef struct sblas_opts {
                    /* 4 row blocks stored sorted by column (x read in column */
                    /* order, products put back in element order in LDS), 6  */
                    /* row blocks over column segments (each row cut at      */
-                   /* column-segment edges, about 4 MB of x per segment; the */
+                   /* column-segment edges, about 2 MB of x per segment; the */
                    /* pieces summed segment by segment, then merged per row) */
   int items;       /* row-block form: elements per thread (8 or 16; 0 auto) */
   int panel_rows;  /* column-panel form: rows per row panel (1..2048, or   */
