@@ -51,6 +51,16 @@ Choice decide(const RowStats &s, int64_t nnz) {
   return c;
 }
 
+// Appends a vector of n entries to a list for the FP32 conversion: its chunks
+// of kX32Chunk entries follow the list's chunks so far.
+void add_x32(std::vector<sblas::BatchX32> &segs, std::vector<int> &chunks, const double *x,
+             float *x32, int64_t n) {
+  const int seg = static_cast<int>(segs.size());
+  segs.push_back(sblas::BatchX32{x, x32, n, static_cast<int>(chunks.size())});
+  const int64_t count = (n + sblas::kX32Chunk - 1) / sblas::kX32Chunk;
+  chunks.insert(chunks.end(), static_cast<size_t>(count), seg);
+}
+
 }  // namespace
 }  // namespace sblas
 
@@ -74,12 +84,14 @@ struct sblas_batch_s {
   // launch ahead of the row blocks
   std::vector<sblas::BatchX32> xsegs;
   sblas::BatchX32 *d_xsegs = nullptr;
-  int64_t x32_total = 0;
+  std::vector<int> xchunks;  // segment of each chunk of the conversion
+  int *d_xchunks = nullptr;
   // and those of the matrices with launches of their own (one launch ahead
   // of theirs)
   std::vector<sblas::BatchX32> solo_xsegs;
   sblas::BatchX32 *d_solo_xsegs = nullptr;
-  int64_t solo_x32_total = 0;
+  std::vector<int> solo_xchunks;
+  int *d_solo_xchunks = nullptr;
   // item list: [0, n_panel) row panels in row order, then row blocks
   int n_panel = 0;
   int grid_panel = 0, grid_rows = 0;
@@ -134,6 +146,13 @@ struct sblas_batch_s {
     // with one kind only, it runs on s itself
     hipStream_t ps = (used > 1 && !solo.empty()) ? side2 : s;
     hipStream_t rs = (used > 1) ? side : s;
+    // the FP32 copies of x of the matrices with launches of their own come
+    // first, alone on the GPU (0.04 ms for the 64-matrix FP32 batch): behind
+    // the fork they shared the GPU with the persistent panel launch and the
+    // first of those launches began only after 0.23 ms
+    int st = sblas::launch_batch_x32(d_solo_xsegs, d_solo_xchunks,
+                                     static_cast<int>(solo_xchunks.size()), s);
+    if (st != SBLAS_OK) return st;
     if (used > 1) {
       SBLAS_HIP_TRY(hipEventRecord(fork, s));
       if (grid_panel > 0 && ps != s) SBLAS_HIP_TRY(hipStreamWaitEvent(ps, fork, 0));
@@ -142,24 +161,20 @@ struct sblas_batch_s {
     // beside the matrices with launches of their own, the row blocks run one
     // item per workgroup, so CUs pass to those launches as items finish (a
     // persistent launch held every CU for 1.9 ms while they waited: 64-matrix
-    // FP32 batch 4.66 ms, 4.48 ms this way); SBLAS_BATCH_FLAT overrides (0
+    // FP32 batch 4.66 ms, 4.49 ms this way); SBLAS_BATCH_FLAT overrides (0
     // persistent, 1 flat row blocks, 2 flat row panels too)
     int flat = sblas::env_int_b("SBLAS_BATCH_FLAT", -1);
     if (flat < 0) flat = solo.empty() ? 0 : 1;
-    int st = sblas::launch_batch(sblas::kBatchPanel, items_per_thread, grid_panel, n_panel,
+    st = sblas::launch_batch(sblas::kBatchPanel, items_per_thread, grid_panel, n_panel,
                                  d_items, d_mats, d_partials, d_qbound, d_ctr, ps, flat >= 2);
     if (st != SBLAS_OK) return st;
-    st = sblas::launch_batch_x32(static_cast<int>(xsegs.size()), d_xsegs, x32_total, rs);
+    st = sblas::launch_batch_x32(d_xsegs, d_xchunks, static_cast<int>(xchunks.size()), rs);
     if (st != SBLAS_OK) return st;
     st = sblas::launch_batch(sblas::kBatchRowBlock, items_per_thread, grid_rows, n_rows,
                              d_items + n_panel, d_mats, d_partials, d_qbound + Q, d_ctr + Q, rs,
                              flat >= 1);
     if (st != SBLAS_OK) return st;
-    // their FP32 copies of x in one launch, then one launch each
-    st = sblas::launch_batch_x32(static_cast<int>(solo_xsegs.size()), d_solo_xsegs,
-                                 solo_x32_total, s);
-    if (st != SBLAS_OK) return st;
-    for (int i : solo) {
+    for (int i : solo) {  // one launch each
       const sblas::BatchMatrix &M = mats[i];
       st = sblas::launch_panel_spmv(M.val_type, panels[i], M.m, M.n, 1, M.x, M.y, M.alpha, M.beta,
                                     s, true);
@@ -368,8 +383,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
           return fail(SBLAS_ERR_HIP);
         }
         f.x32 = static_cast<float *>(p);
-        b->solo_xsegs.push_back(BatchX32{it.d_x, f.x32, b->solo_x32_total});
-        b->solo_x32_total += it.n;
+        add_x32(b->solo_xsegs, b->solo_xchunks, it.d_x, f.x32, it.n);
       }
     }
     b->mats.push_back(M);
@@ -400,8 +414,7 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
         }
         b->bufs.push_back(p);
         b->mats.back().x32 = static_cast<const float *>(p);
-        b->xsegs.push_back(BatchX32{it.d_x, static_cast<float *>(p), b->x32_total});
-        b->x32_total += it.n;
+        add_x32(b->xsegs, b->xchunks, it.d_x, static_cast<float *>(p), it.n);
       }
       const std::vector<int64_t> bounds =
           element_bounds(SBLAS_PART_V1, it.m, h_rp.data(), it.nnz, c.chunks, 1);
@@ -502,7 +515,9 @@ int sblas_batch_create(int count, const sblas_batch_item *items, const sblas_opt
       (st = b->upload(&b->d_gptr, gptr)) != SBLAS_OK ||
       (st = b->upload(&b->d_gslot, gslot)) != SBLAS_OK ||
       (st = b->upload(&b->d_xsegs, b->xsegs)) != SBLAS_OK ||
-      (st = b->upload(&b->d_solo_xsegs, b->solo_xsegs)) != SBLAS_OK)
+      (st = b->upload(&b->d_xchunks, b->xchunks)) != SBLAS_OK ||
+      (st = b->upload(&b->d_solo_xsegs, b->solo_xsegs)) != SBLAS_OK ||
+      (st = b->upload(&b->d_solo_xchunks, b->solo_xchunks)) != SBLAS_OK)
     return fail(st);
   std::vector<double> zeros(static_cast<size_t>(std::max<int64_t>(nslots, 1)), 0.0);
   if ((st = b->upload(&b->d_partials, zeros)) != SBLAS_OK) return fail(st);

@@ -297,10 +297,13 @@ struct BatchMatrix {
 struct BatchX32 {
   const double *x;
   float *x32;
-  int64_t start;
+  int64_t n;        // entries
+  int first_chunk;  // its first chunk of kX32Chunk entries in the launch
 };
-// x32[i] = x[i] rounded to FP32 for every segment (total entries in all).
-int launch_batch_x32(int nseg, const BatchX32 *segs, int64_t total, hipStream_t s);
+constexpr int kX32Chunk = 2048;  // entries per workgroup of the conversion
+// x32[i] = x[i] rounded to FP32 for every segment; chunk c (one workgroup)
+// is chunk c - first_chunk of segment chunk_seg[c].
+int launch_batch_x32(const BatchX32 *segs, const int *chunk_seg, int nchunks, hipStream_t s);
 constexpr int kBatchRowBlock = 0;
 constexpr int kBatchPanel = 1;
 struct BatchItem {  // one row block, or one row panel, of one matrix
