@@ -1026,8 +1026,11 @@ int Plan::choose_csr(const std::vector<int> &h_rp) {
   if ((st = setup_csr5(h_rp)) != SBLAS_OK) return st;
   std::vector<std::pair<int, int>> forms{
       {SBLAS_KERNEL_VECTOR, 2}, {SBLAS_KERNEL_CSR5, 2}, {SBLAS_KERNEL_VECTOR, 4}};
-  // column segments where x is several times an XCD's L2 (4 MB)
-  if (static_cast<int64_t>(n) * 8 >= (int64_t{16} << 20)) {
+  // column segments where x is several times an XCD's L2 (4 MB), unless the
+  // format build's count tables (16 bytes per segment and row) would pass
+  // 4 GB
+  if (static_cast<int64_t>(n) * 8 >= (int64_t{16} << 20) &&
+      static_cast<int64_t>(segments_for(n)) * m <= (int64_t{1} << 28)) {
     st = setup_segments(segments_for(n));
     if (st == SBLAS_ERR_ALLOC || st == SBLAS_ERR_UNSUPPORTED) {
       clear_error();
