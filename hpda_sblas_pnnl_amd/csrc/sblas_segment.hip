@@ -124,9 +124,10 @@ __global__ void k_seg_tab(int nb, int m, int P, const int *__restrict__ vindex,
 // no elements in the segment; then each thread adds its row's entries,
 // segments ascending, to a sum that starts from 0 (adding a zero entry leaves
 // the sum's bits as they are), and writes y[r] = alpha * sum + beta * y[r].
-// Each thread issues the loads of up to kSegUnroll list entries before it
-// waits for any.  The kernel waits on its dependent loads (run bounds, then
-// runs), so it wants many workgroups per CU: at 8 segments a group the table
+// The run bounds of all segments are loaded once, at the start, and each
+// thread issues the loads of up to kSegUnroll list entries before it waits
+// for any.  The kernel waits on its dependent loads (run bounds, then runs),
+// so it wants many workgroups per CU: at 8 segments a group the table
 // takes 16 KB and eight workgroups fit (on R-MAT scale 22 at 16 segments the
 // whole product ran 0.402 ms; 16 segments a group, 32 KB: 0.429 ms; one
 // barrier per segment over LDS sums: 0.417 ms; persistent workgroups loading
@@ -137,25 +138,25 @@ __global__ __launch_bounds__(kSegMergeRows) void k_seg_merge(
     int m, int P, const int *__restrict__ vtab, const uint8_t *__restrict__ vrow,
     const double *__restrict__ yv, double *__restrict__ y, double alpha, double beta) {
   __shared__ double tab[kSegGroup][kSegMergeRows];
-  __shared__ int run_lo[kSegGroup], run_len[kSegGroup];
+  __shared__ int run_lo[kSegmentsMax], run_len[kSegmentsMax];
   const int t = threadIdx.x, b = blockIdx.x;
   const int r = b * kSegMergeRows + t;
   const double y_old = (beta != 0.0 && r < m) ? y[r] : 0.0;
+  if (t < P) {  // every segment's run bounds at once, read after the first barrier
+    const int lo = vtab[b * P + t];
+    run_lo[t] = lo;
+    run_len[t] = vtab[(b + 1) * P + t] - lo;
+  }
   double acc = 0.0;
   for (int s0 = 0; s0 < P; s0 += kSegGroup) {
     const int g = min(kSegGroup, P - s0);
-    if (t < g) {
-      const int lo = vtab[b * P + s0 + t];
-      run_lo[t] = lo;
-      run_len[t] = vtab[(b + 1) * P + s0 + t] - lo;
-    }
 #pragma unroll
     for (int k = 0; k < kSegGroup; k++) tab[k][t] = 0.0;
     __syncthreads();
     int pre[kSegGroup + 1];  // list position where each run starts
     pre[0] = 0;
 #pragma unroll
-    for (int k = 0; k < kSegGroup; k++) pre[k + 1] = pre[k] + (k < g ? run_len[k] : 0);
+    for (int k = 0; k < kSegGroup; k++) pre[k + 1] = pre[k] + (k < g ? run_len[s0 + k] : 0);
     const int total = pre[kSegGroup];
     for (int j0 = t; j0 < total; j0 += kSegUnroll * kSegMergeRows) {
       int seg[kSegUnroll], idx[kSegUnroll];
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(kSegMergeRows) void k_seg_merge(
             base = pre[q];
           }
         seg[u] = j < total ? k : -1;
-        idx[u] = run_lo[k] + (jj - base);
+        idx[u] = run_lo[s0 + k] + (jj - base);
       }
       uint8_t row[kSegUnroll];
       double v[kSegUnroll];
