@@ -380,6 +380,32 @@ int Plan::setup_stream(const int *h_rp) {
     set_error(SBLAS_ERR_ALLOC, "plan: partials allocation failed");
     return SBLAS_ERR_ALLOC;
   }
+  // Multi-vector products on large matrices with irregular rows take blocks
+  // of 16 elements a thread: R-MAT scale 22, k = 2 / 4 / 8 / 16: 0.81 / 1.18 /
+  // 1.52 / 2.22 ms against 0.91 / 1.32 / 1.67 / 2.55 with 8.  A banded matrix
+  // of 5.6e7 elements (row CV 0.3) ran 1 % slower with them, the power-law
+  // one of 3.2e5 elements 1.6x slower (too few blocks).
+  if (opts.items == 0 && nnz >= (int64_t{1} << 23) && stats.cv >= 1.0) {
+    const std::vector<RowBlock> kb = build_row_blocks(exec, h_rp, kStreamBlock * 16);
+    if ((st = upload(*this, &d_kblocks, kb)) != SBLAS_OK) return st;
+    as_chunks.clear();
+    for (const RowBlock &b : kb) as_chunks.push_back(ExecChunk{b.lo, b.hi, b.row_begin, b.row_end});
+    merge_lists(as_chunks, h_rp, mrows, mptr, mslots);
+    kgroups = static_cast<int>(mrows.size());
+    if ((st = upload(*this, &d_kmrows, mrows)) != SBLAS_OK) return st;
+    if ((st = upload(*this, &d_kmptr, mptr)) != SBLAS_OK) return st;
+    if ((st = upload(*this, &d_kmslots, mslots)) != SBLAS_OK) return st;
+    d_kpartials = static_cast<double *>(
+        dev_alloc(sizeof(double) * 2 * max_k * std::max<size_t>(kb.size(), 1)));
+    if (!d_kpartials) {
+      set_error(SBLAS_ERR_ALLOC, "plan: partials allocation failed");
+      return SBLAS_ERR_ALLOC;
+    }
+    slk = sl;
+    slk.items = 16;
+    slk.nblocks = static_cast<int>(kb.size());
+    slk.grid = slk.xcd_remap ? ((slk.nblocks + 7) / 8) * 8 : slk.nblocks;
+  }
   return SBLAS_OK;
 }
 
@@ -503,15 +529,25 @@ int Plan::run_spmm(int k, double alpha, const double *X, double beta, double *Y,
     return SBLAS_OK;
   }
   if (variant >= 2 && k > 1 && streams.empty()) {
+    // k lanes per row unless a sub-wave was asked for (banded, k = 4: 0.536
+    // against 0.567 ms with the k = 1 sub-wave of 32; R-MAT, k = 2: 0.81
+    // against 0.84 with 4; equal elsewhere)
+    const bool own = slk.grid > 0;
+    StreamLaunch L = own ? slk : sl;
+    if (opts.subwave <= 0) L.subwave = k;
     hipEvent_t *te = timing_slot();
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[0], s));
-    st = launch_csr_blocks_mrhs(val_type, k, sl, d_rblocks, a0, a1, val, X, Y, alpha, beta,
-                                d_bpartials, s);
+    st = launch_csr_blocks_mrhs(val_type, k, L, own ? d_kblocks : d_rblocks, a0, a1, val, X, Y,
+                                alpha, beta, own ? d_kpartials : d_bpartials, s);
     if (st != SBLAS_OK) return st;
     if (te) SBLAS_HIP_TRY(hipEventRecord(te[1], s));
     launches++;
-    if (bgroups > 0 && merge_enabled) {
-      st = launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, k, Y, alpha, beta, s);
+    const int groups = own ? kgroups : bgroups;
+    if (groups > 0 && merge_enabled) {
+      st = own ? launch_merge(d_kmrows, d_kmptr, d_kmslots, kgroups, d_kpartials, k, Y, alpha,
+                              beta, s)
+               : launch_merge(d_bmrows, d_bmptr, d_bmslots, bgroups, d_bpartials, k, Y, alpha,
+                              beta, s);
       if (st != SBLAS_OK) return st;
       launches++;
     }
@@ -1069,7 +1105,8 @@ bool Plan::has_merge(int k) const {
   if (variant == 6 && k == 1 && streams.empty()) return false;  // merges run inside the product
   if (variant == 3 && k == 1 && streams.empty()) return pgroups > 0;
   if (variant == 4 && k == 1 && streams.empty()) return sgroups > 0;
-  if (variant >= 2 && streams.empty()) return bgroups > 0;  // row blocks (any k)
+  if (variant >= 2 && streams.empty())  // row blocks (any k)
+    return (k > 1 && slk.grid > 0) ? kgroups > 0 : bgroups > 0;
   return ngroups > 0;
 }
 
@@ -1083,6 +1120,10 @@ int Plan::run_merge(int k, double alpha, double beta, double *Y, hipStream_t s) 
                        : SBLAS_OK;
   if (variant == 4 && k == 1 && streams.empty())
     return sgroups > 0 ? launch_merge(d_smrows, d_smptr, d_smslots, sgroups, sorted.partials, 1, Y,
+                                      alpha, beta, s)
+                       : SBLAS_OK;
+  if (variant >= 2 && streams.empty() && k > 1 && slk.grid > 0)
+    return kgroups > 0 ? launch_merge(d_kmrows, d_kmptr, d_kmslots, kgroups, d_kpartials, k, Y,
                                       alpha, beta, s)
                        : SBLAS_OK;
   if (variant >= 2 && streams.empty())

@@ -67,6 +67,14 @@ struct Plan {
   int *d_bmrows = nullptr, *d_bmptr = nullptr, *d_bmslots = nullptr;
   int bgroups = 0;
   double *d_bpartials = nullptr;
+  // multi-vector products over row blocks of 16 elements a thread where
+  // those pay (large matrices with irregular rows); grid 0 otherwise, and
+  // the row blocks above serve them
+  StreamLaunch slk{};
+  RowBlock *d_kblocks = nullptr;
+  int *d_kmrows = nullptr, *d_kmptr = nullptr, *d_kmslots = nullptr;
+  int kgroups = 0;
+  double *d_kpartials = nullptr;
 
   // segmented tiles (COO / merge path / CSR5)
   int tiles = 0;
