@@ -245,6 +245,9 @@ struct PanelFormat {
   // narrow-x form (FP32 / BF16 values): FP32 copy of x written before each
   // launch, padded with zeros to whole column panels; nullptr: FP64 x
   float *x32 = nullptr;
+  // BF16 values in the sorted-group layout: value (high half) and column
+  // offset (low half) of each element in one word, in place of pval / pcol
+  uint32_t *packed = nullptr;
   int64_t padded = 0;           // elements stored, slices padded
   std::vector<ExecChunk> hpanels;  // host copy: element window and rows of each panel
   std::vector<void *> buffers;

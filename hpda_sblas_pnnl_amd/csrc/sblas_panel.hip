@@ -107,6 +107,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                    x32, y, alpha, beta, partials + 2 * p, lds);
 }
 
+// BF16 elements of the sorted-group layout packed: value << 16 | column offset
+__global__ void k_pack_bf16(int64_t count, const uint16_t *__restrict__ v,
+                            const uint16_t *__restrict__ c, uint32_t *__restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (static_cast<uint32_t>(v[i]) << 16) | c[i];
+}
+
 // x32[i] = x[i] rounded to FP32 (the narrow-x column-panel form's x copy)
 __global__ void k_x_to_f32(int n, const double *__restrict__ x, float *__restrict__ x32) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -620,6 +627,22 @@ int panel_build(int val_type, int rows, int m, int n, int64_t nnz, const int *rp
     release(f.srp32);
     f.srp32 = nullptr;
   }
+  if (sorted && val_type == SBLAS_VAL_BF16) {
+    // each element's value and column offset into one word
+    f.packed = static_cast<uint32_t *>(alloc(sizeof(uint32_t) * std::max<int64_t>(f.padded, 1)));
+    if (!f.packed) {
+      set_error(SBLAS_ERR_ALLOC, "panel: packed BF16 elements");
+      return SBLAS_ERR_ALLOC;
+    }
+    hipLaunchKernelGGL(k_pack_bf16, dim3(static_cast<unsigned>((f.padded + 255) / 256)), dim3(256),
+                       0, s, f.padded, static_cast<const uint16_t *>(f.pval), f.pcol, f.packed);
+    SBLAS_HIP_TRY(hipGetLastError());
+    SBLAS_HIP_TRY(hipStreamSynchronize(s));
+    release(f.pval);
+    release(f.pcol);
+    f.pval = nullptr;
+    f.pcol = nullptr;
+  }
   *chosen = true;
   return SBLAS_OK;
 }
@@ -667,16 +690,18 @@ int panel_launch_nx(const PanelFormat &f, int m, int n, int xcd_remap, const dou
                                                       beta, s);
 }
 
+// VT = uint32_t: packed BF16 elements (f.packed)
 template <typename VT, bool NX>
 int sorted_launch(const PanelFormat &f, int n, int xcd_remap, const double *x, double *y,
                   double alpha, double beta, hipStream_t s) {
+  const VT *vals = sorted_panel::kPackedBf16<VT> ? reinterpret_cast<const VT *>(f.packed)
+                                                 : static_cast<const VT *>(f.pval);
   const int grid = xcd_remap ? ((f.npanels + 7) / 8) * 8 : f.npanels;
 #define SBLAS_SORTED_CASE(RPT)                                                                \
   case RPT:                                                                                   \
     hipLaunchKernelGGL((k_panel_sorted<VT, RPT, NX>), dim3(grid), dim3(256), 0, s, n, f.rows,  \
                        f.npanels, xcd_remap, f.info, f.slice_base, f.slice_off, f.perm,       \
-                       f.goff, static_cast<const VT *>(f.pval), f.pcol, x, f.x32, y, alpha,   \
-                       beta, f.partials);                                                     \
+                       f.goff, vals, f.pcol, x, f.x32, y, alpha, beta, f.partials);           \
     break;
   switch (f.rpt) {
     SBLAS_SORTED_CASE(1)
@@ -703,11 +728,15 @@ int panel_launch_vt(const PanelFormat &f, int m, int n, int xcd_remap, const dou
         hipLaunchKernelGGL(k_x_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, x, f.x32);
         SBLAS_HIP_TRY(hipGetLastError());
       }
+      if (f.layout == kPanelSorted && f.packed)
+        return sorted_launch<uint32_t, true>(f, n, xcd_remap, x, y, alpha, beta, s);
       if (f.layout == kPanelSorted)
         return sorted_launch<VT, true>(f, n, xcd_remap, x, y, alpha, beta, s);
       return panel_launch_nx<VT, true>(f, m, n, xcd_remap, x, y, alpha, beta, s);
     }
   }
+  if (f.layout == kPanelSorted && f.packed)
+    return sorted_launch<uint32_t, false>(f, n, xcd_remap, x, y, alpha, beta, s);
   if (f.layout == kPanelSorted)
     return sorted_launch<VT, false>(f, n, xcd_remap, x, y, alpha, beta, s);
   return panel_launch_nx<VT, false>(f, m, n, xcd_remap, x, y, alpha, beta, s);
@@ -737,8 +766,8 @@ int sorted_occupancy_rpt(int val_type, bool narrow_x) {
       : val_type == SBLAS_VAL_F32
           ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_sorted<float, RPT, true>)
                       : reinterpret_cast<const void *>(&k_panel_sorted<float, RPT, false>))
-          : (narrow_x ? reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, RPT, true>)
-                      : reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, RPT, false>));
+          : (narrow_x ? reinterpret_cast<const void *>(&k_panel_sorted<uint32_t, RPT, true>)
+                      : reinterpret_cast<const void *>(&k_panel_sorted<uint32_t, RPT, false>));
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
       per_cu < 1)
@@ -767,8 +796,8 @@ int panel_rows_for(int val_type, int m, int device, bool narrow_x, int layout, i
                  ? (narrow_x ? reinterpret_cast<const void *>(&k_panel_sorted<float, 8, true>)
                              : reinterpret_cast<const void *>(&k_panel_sorted<float, 8, false>))
                  : (narrow_x
-                        ? reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, 8, true>)
-                        : reinterpret_cast<const void *>(&k_panel_sorted<uint16_t, 8, false>)))
+                        ? reinterpret_cast<const void *>(&k_panel_sorted<uint32_t, 8, true>)
+                        : reinterpret_cast<const void *>(&k_panel_sorted<uint32_t, 8, false>)))
       : val_type == SBLAS_VAL_F64
           ? reinterpret_cast<const void *>(&k_panel_spmv<double, 8, uint16_t, false>)
       : val_type == SBLAS_VAL_F32

@@ -25,7 +25,14 @@
 // order (test/dspmv_test_smt.cu:45-51).  A padding element adds -0.0, which
 // leaves every sum unchanged.  With NX (FP32 / BF16 values, x staged from its
 // FP32 copy) products are rounded to FP32 and added in FP64.
+//
+// BF16 values come packed (VT = uint32_t): each element's value in the high
+// half of a word and its column offset in the low half, so a lane issues one
+// 4-byte load per element instead of two 2-byte loads (banded n = 2e6, 8
+// chunks: 0.299 against 0.314 ms).
 #pragma once
+
+#include <type_traits>
 
 #include "internal.hpp"
 #include "panel_device.hpp"
@@ -50,16 +57,24 @@ __host__ __device__ constexpr int unroll_for(int rpt) {
   return rpt <= 1 ? 16 : rpt == 2 ? 8 : rpt <= 4 ? 4 : 3;
 }
 
+// a packed BF16 element (see above): its value
+template <typename VT>
+constexpr bool kPackedBf16 = std::is_same<VT, uint32_t>::value;
+
 template <typename VT>
 __device__ __forceinline__ double widen1(VT v) {
-  if constexpr (sizeof(VT) == 2)
+  if constexpr (kPackedBf16<VT>)
+    return static_cast<double>(__uint_as_float(static_cast<uint32_t>(v) & 0xFFFF0000u));
+  else if constexpr (sizeof(VT) == 2)
     return static_cast<double>(__uint_as_float(static_cast<uint32_t>(v) << 16));
   else
     return static_cast<double>(v);
 }
 template <typename VT>
 __device__ __forceinline__ float widen1_f(VT v) {
-  if constexpr (sizeof(VT) == 2)
+  if constexpr (kPackedBf16<VT>)
+    return __uint_as_float(static_cast<uint32_t>(v) & 0xFFFF0000u);
+  else if constexpr (sizeof(VT) == 2)
     return __uint_as_float(static_cast<uint32_t>(v) << 16);
   else
     return static_cast<float>(v);
@@ -185,15 +200,20 @@ __device__ __forceinline__ void panel(int p, int n, int R, const PanelInfo &pi,
           if (k + kk < len[j]) {
             const int e = gb[j] + 64 * (k + kk);
             v[j][kk] = __builtin_nontemporal_load(vs + e);
-            c[j][kk] = __builtin_nontemporal_load(cs + e);
+            if constexpr (!kPackedBf16<VT>) c[j][kk] = __builtin_nontemporal_load(cs + e);
           }
 #pragma unroll
       for (int j = 0; j < RPT; j++)
 #pragma unroll
         for (int kk = 0; kk < KU; kk++)
           if (k + kk < len[j]) {
-            const int ci = c[j][kk] & (W - 1);
-            const bool pad = c[j][kk] == kSortedPad;
+            // the column of a packed element is split off only here, where
+            // the product needs it
+            const uint16_t cv = kPackedBf16<VT>
+                                    ? static_cast<uint16_t>(static_cast<uint32_t>(v[j][kk]))
+                                    : c[j][kk];
+            const int ci = cv & (W - 1);
+            const bool pad = cv == kSortedPad;
             if constexpr (NX) {
               const float pr = panel::mul_rn_f(widen1_f(v[j][kk]),
                                                reinterpret_cast<const float *>(buf)[ci]);
