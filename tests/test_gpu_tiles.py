@@ -148,7 +148,8 @@ def test_panel_form_bitwise(oracle, hw, layout):
     """Column-panel form of kernel 1, rows in order (layout 1) or sorted by
     length in groups of 64 (layout 2): each row is summed by one thread or
     lane in ascending column order with separately rounded operations, so it
-    equals the host oracle bit for bit."""
+    equals the host oracle bit for bit, with FP64, FP32 or BF16 values and x
+    staged in FP64."""
     import torch
     from hpda_sblas_pnnl_amd import gen as G
     from hpda_sblas_pnnl_amd.device import SpmvPlan
@@ -173,6 +174,16 @@ def test_panel_form_bitwise(oracle, hw, layout):
     plan32.spmv(dx, dy)
     torch.cuda.synchronize()
     ref = oracle.csr_spmv_f32val(A.m, v32, A.row_ptr, A.col, x, 1.0, 0.0, np.zeros(A.m))
+    assert np.array_equal(dy.cpu().numpy(), ref)
+    # BF16 values likewise (in layout 2 each element's value and column offset
+    # packed into one word)
+    bits = G.to_bf16_bits(A.val)
+    plan16 = SpmvPlan("csr", A.m, A.n, drp, dcol, torch.from_numpy(bits.view(np.int16)).cuda(),
+                      variant=3, narrow_x=1, panel_layout=layout)
+    dy = torch.from_numpy(y0.copy()).cuda()
+    plan16.spmv(dx, dy, 0.5, -1.5)
+    torch.cuda.synchronize()
+    ref = oracle.csr_spmv_bf16val(A.m, bits, A.row_ptr, A.col, x, 0.5, -1.5, y0)
     assert np.array_equal(dy.cpu().numpy(), ref)
 
 
